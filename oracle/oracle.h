@@ -1,0 +1,136 @@
+/*
+ * oracle.h — CPU oracle for the curvilinear velocity-stress solver.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * algorithm (XiaoYu-1111/SeisWaveLab--Curvilinear 0.17.0) used by tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker.
+ * The product (seiswavelab--curvilinear_amd/) never links, loads or calls it.
+ *
+ * Parameter blocks mirror Seis2D::Curvilinear::Parameters
+ * (include/Seis2D/CurvilinearSolver2D.hpp:29-57) and
+ * Seis3D::Curvilinear::SimulationParameters (include/Seis3D/CurvilinearSolver.hpp:70-95)
+ * field for field, plus an optional per-node material override that the
+ * reference's 2D API cannot express (heterogeneous models).  The same
+ * layout is used by the product header include/swl.h so one ctypes
+ * structure can drive the product, the oracle and the reference build.
+ */
+#ifndef SWL_ORACLE_H
+#define SWL_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAX_LAYERS 16
+
+typedef struct orc_params2d {
+    int precision;            /* 0 = FP64, 1 = FP32 */
+    int nx, nz;
+    double spacing, vp, vs, density;
+    double relief, wavelength, flatten_depth, frequency, duration, cfl, force_per_length;
+    int source_x, source_z, max_steps;
+    int source_kind;          /* 0 directional force, 1 isotropic moment */
+    double source_direction[3];
+    double moment_per_length;
+    double source_sigma_m, source_radius_m;
+    int absorbing;
+    int sponge_x, sponge_z;
+    double sponge_strength;
+    int absorber;             /* 0 sponge, 1 CFS-PML */
+    double pml_reflection, pml_kappa_max, pml_frequency;
+    int external_absorption;
+    int receiver_start, receiver_count, receiver_stride, receiver_depth;
+    double sample_interval_ms;
+    int layered, n_layers;
+    double layer_thickness[ORC_MAX_LAYERS], layer_vp[ORC_MAX_LAYERS], layer_vs[ORC_MAX_LAYERS],
+        layer_density[ORC_MAX_LAYERS], layer_qp[ORC_MAX_LAYERS], layer_qs[ORC_MAX_LAYERS];
+    int attenuation, mechanisms;
+    double q_min_frequency, q_max_frequency, q_reference_frequency, qp, qs;
+    int imported_surface;
+    double surface_datum;
+    int n_surface;
+    const double *surface_xz;             /* n_surface (x, elevation) pairs */
+    /* Optional heterogeneous model on the logical nx*nz grid (X fastest).  When set,
+       they replace the layered/homogeneous Vp, Vs, density per node; absorber strips
+       and ghosts take the nearest logical node.  Not expressible in the reference. */
+    const double *node_vp, *node_vs, *node_density;
+    int threads;                           /* CPU workers; 0 = min(8, procs) */
+} orc_params2d;
+
+typedef struct orc_params3d {
+    int nx, ny, nz;
+    double spacing, vp, vs, density;
+    int layered, layer_depth_mode, n_layers; /* mode: 0 GridIndex, 1 BelowSurface, 2 Datum */
+    double layer_thickness[8], layer_vp[8], layer_vs[8], layer_density[8];
+    double dominant_frequency;
+    int source_x, source_y, source_z;
+    int source_kind;
+    double source_direction[3];
+    double source_force, source_moment;
+    double duration;
+    int max_steps;
+    double cfl;
+    double topography_amplitude, topography_wavelength, terrain_transition_depth;
+    /* Surface DEM (row-major [y][x]); empty (dem_nx == 0) selects analytic terrain. */
+    int dem_nx, dem_ny;
+    double dem_spacing, dem_origin_x, dem_origin_y, dem_datum;
+    const double *dem_elevations;
+    /* Receiver array (ReceiverArraySettings3D): kind 0 AutoX, 1 X, 2 Y, 3 GridXY. */
+    int receiver_kind, receiver_start_x, receiver_start_y, receiver_z;
+    int receiver_count_x, receiver_count_y, receiver_stride_x, receiver_stride_y;
+    int absorbing, absorbing_cells[3];
+    double absorbing_strength;
+    int threads;
+} orc_params3d;
+
+/* 2D session (names mirror Seis2D::Curvilinear::Simulation). */
+void *orc2d_create(const orc_params2d *p, char *error, int error_len);
+void orc2d_destroy(void *h);
+int orc2d_advance(void *h, int count, char *error, int error_len);
+int orc2d_reset(void *h);
+int orc2d_step(void *h);
+int orc2d_total(void *h);
+double orc2d_dt(void *h);
+double orc2d_record_dt(void *h);
+/* layout: nx_pad, nz_pad, offset_x, cpml_width_x, cpml_width_z, fields, cpml_count, receivers, record_len, mechanisms */
+void orc2d_layout(void *h, int64_t *out);
+/* diagnostics: energy, peak, max_velocity, traction, finite, wave_dt, damping_dt, source_dt, cpml_dt,
+   relaxation_dt, ppw, max_q_error, max_metric, min_jacobian, max_edge, flatten_depth,
+   source_weight_sum, source_rms_radius */
+void orc2d_diagnostics(void *h, double *out);
+/* receivers: vx[r*len+n], vz[r*len+n], positions (x,z) per receiver, logical (x,z) per receiver */
+void orc2d_receivers(void *h, double *vx, double *vz, double *position, int32_t *xz);
+void orc2d_probe_indices(void *h, int64_t *out);
+/* field f of the accepted state (padded storage, promoted to double) */
+void orc2d_field(void *h, int f, double *out);
+void orc2d_pml(void *h, double *out);
+/* model arrays: 0 a,1 b,2 jac,3 lambda,4 mu,5 rho,6 source,7 damping,8 grad x,9 grad z,
+   10 normals(2N),11 coordinates(2N),12 vp,13 vs,14 cpmlX(3nx),15 cpmlZ(3nz),
+   16..18 deltaP, 19..21 deltaS (all FP64 construction values) */
+int orc2d_model(void *h, int which, double *out);
+int64_t orc2d_pml_slot(int nx_pad, int nz_pad, int width_x, int width_z, int64_t q, int axis);
+size_t orc2d_params_size(void);
+
+void *orc3d_create(const orc_params3d *p, char *error, int error_len);
+void orc3d_destroy(void *h);
+int orc3d_advance(void *h, int count, char *error, int error_len);
+int orc3d_step(void *h);
+int orc3d_total(void *h);
+double orc3d_dt(void *h);
+/* layout: nx_pad, ny_pad, nz_pad, receivers, record_len */
+void orc3d_layout(void *h, int64_t *out);
+/* diagnostics: energy, peak, source_peak, max_velocity, traction_rate_relative, finite,
+   min_jacobian, max_metric_norm, min_cell_length, max_cell_edge, wave_dt, damping_dt, source_dt */
+void orc3d_diagnostics(void *h, double *out);
+void orc3d_receivers(void *h, double *vx, double *vy, double *vz, double *position, int64_t *index);
+void orc3d_field(void *h, int f, double *out);
+/* 0 metrics(9N), 1 jacobian, 2 normals(3N), 3 coordinates(3N), 4 lambda, 5 mu, 6 density,
+   7 damping, 8 source, 9..11 source gradient */
+int orc3d_model(void *h, int which, double *out);
+size_t orc3d_params_size(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
