@@ -1,0 +1,221 @@
+/*
+ * swl.h — C-ABI of the MI355X-native curvilinear velocity-stress solver (libswl.so).
+ *
+ * Every entry point returns an int status (0 = ok, nonzero = error) unless it
+ * returns a value; on error the message is available from swl_last_error()
+ * (thread-local).  No C++ types or exceptions cross this boundary; all arrays are
+ * plain host pointers with explicit sizes.  The engine owns every device
+ * allocation; host arrays passed in are copied during the call.
+ *
+ * Three layers are exported:
+ *  1. Engines (swl_engine*): the HIP backends that replace the reference's
+ *     CudaEngine (src/two_d/CudaEngine2D.hpp:5-24) and CudaEngine3D
+ *     (src/three_d/CudaEngine3D.hpp:7-24).  They take an already-built FP64 mesh.
+ *  2. Sessions (swl_sim2d* / swl_sim3d*): the 2D Simulation
+ *     (include/Seis2D/CurvilinearSolver2D.hpp:93-123) and 3D ISimulationBackend3D
+ *     (include/Seis3D/CurvilinearSolver.hpp:165-185) façades in C++20, which build
+ *     geometry, materials, absorbers, sources, the dt plan and receivers exactly as
+ *     the reference does and drive an engine.
+ *  3. Batches (swl_batch2d*): many independent 2D shots stepped by one launch per
+ *     RK stage (no counterpart in the reference; used by the serving loop).
+ */
+#ifndef SWL_H
+#define SWL_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWL_MAX_LAYERS_2D 16
+#define SWL_MAX_LAYERS_3D 8
+
+/* Mirrors Seis2D::Curvilinear::Parameters (include/Seis2D/CurvilinearSolver2D.hpp:29-57).
+   node_vp/node_vs/node_density: optional heterogeneous model on the logical nx*nz grid
+   (X fastest), an extension the reference 2D API cannot express. */
+typedef struct swl_params2d {
+    int precision; /* 0 = FP64, 1 = FP32 (Seis2D::Curvilinear::Precision) */
+    int nx, nz;
+    double spacing, vp, vs, density;
+    double relief, wavelength, flatten_depth, frequency, duration, cfl, force_per_length;
+    int source_x, source_z, max_steps;
+    int source_kind; /* SeisWave::SourceKind: 0 directional force, 1 isotropic moment */
+    double source_direction[3];
+    double moment_per_length;
+    double source_sigma_m, source_radius_m;
+    int absorbing;
+    int sponge_x, sponge_z;
+    double sponge_strength;
+    int absorber; /* Seis2D::Curvilinear::Absorber: 0 sponge, 1 CFS-PML */
+    double pml_reflection, pml_kappa_max, pml_frequency;
+    int external_absorption;
+    int receiver_start, receiver_count, receiver_stride, receiver_depth;
+    double sample_interval_ms;
+    int layered, n_layers;
+    double layer_thickness[SWL_MAX_LAYERS_2D], layer_vp[SWL_MAX_LAYERS_2D], layer_vs[SWL_MAX_LAYERS_2D],
+        layer_density[SWL_MAX_LAYERS_2D], layer_qp[SWL_MAX_LAYERS_2D], layer_qs[SWL_MAX_LAYERS_2D];
+    int attenuation, mechanisms;
+    double q_min_frequency, q_max_frequency, q_reference_frequency, qp, qs;
+    int imported_surface;
+    double surface_datum;
+    int n_surface;
+    const double *surface_xz;
+    const double *node_vp, *node_vs, *node_density;
+    int threads;
+} swl_params2d;
+
+/* Mirrors Seis3D::Curvilinear::SimulationParameters (include/Seis3D/CurvilinearSolver.hpp:70-95). */
+typedef struct swl_params3d {
+    int nx, ny, nz;
+    double spacing, vp, vs, density;
+    int layered, layer_depth_mode, n_layers;
+    double layer_thickness[SWL_MAX_LAYERS_3D], layer_vp[SWL_MAX_LAYERS_3D], layer_vs[SWL_MAX_LAYERS_3D],
+        layer_density[SWL_MAX_LAYERS_3D];
+    double dominant_frequency;
+    int source_x, source_y, source_z;
+    int source_kind;
+    double source_direction[3];
+    double source_force, source_moment;
+    double duration;
+    int max_steps;
+    double cfl;
+    double topography_amplitude, topography_wavelength, terrain_transition_depth;
+    int dem_nx, dem_ny;
+    double dem_spacing, dem_origin_x, dem_origin_y, dem_datum;
+    const double *dem_elevations;
+    int receiver_kind, receiver_start_x, receiver_start_y, receiver_z;
+    int receiver_count_x, receiver_count_y, receiver_stride_x, receiver_stride_y;
+    int absorbing, absorbing_cells[3];
+    double absorbing_strength;
+    int threads;
+} swl_params3d;
+
+/* ---------------- errors and devices ---------------- */
+/* Message of the last failed call on this thread ("" if none).  Replaces the
+   std::runtime_error("CUDA <action>: <msg>") convention of src/common/CudaSupport.cuh:6-9. */
+const char *swl_last_error(void);
+
+typedef struct swl_device_info {
+    int index;
+    char name[128];
+    char arch[64];
+    uint64_t total_bytes, free_bytes;
+    int compute_units;
+    int available;
+} swl_device_info;
+/* queryCudaDevices3D / queryCudaMemory3D (src/common/CudaDevices.cu:10-61). */
+int swl_device_count(void);
+int swl_device_query(int device, swl_device_info *out);
+
+/* ---------------- 2D engine (replaces CudaEngine, src/two_d/CudaEngine2D.hpp:5-24) ---------------- */
+/* FP64 host mesh in padded storage (Point::MeshT, src/two_d/PointKernels2D.hpp:14-27).
+   The engine converts to the run precision; in FP32 it keeps FP64 copies of the
+   surface row for the traction closure and projection. */
+typedef struct swl_mesh2d_desc {
+    int precision;   /* 0 FP64, 1 FP32 */
+    int nx, nz;      /* padded node counts (computational + 2*3 ghosts) */
+    double h;
+    const double *a, *b, *jac, *normals, *lambda, *mu, *rho, *source, *damping; /* normals: 2 per node */
+    double source_direction[3];
+    const double *source_gradient[2]; /* both null for a directional force */
+    int mechanisms, field_count;
+    const double *cpml_x, *cpml_z; /* 3 per computational node along x / z, or null */
+    int cpml_width_x, cpml_width_z;
+    int64_t cpml_count;
+    double rates[3];
+    const double *delta_p[3], *delta_s[3];
+} swl_mesh2d_desc;
+
+typedef struct swl_diag2d {
+    double energy, max_velocity, traction_relative;
+    int finite;
+    int rhs_registers;
+    uint64_t rhs_local_bytes;
+} swl_diag2d;
+
+typedef struct swl_engine2d swl_engine2d;
+
+/* CudaEngine(const MeshT&, const std::vector<Receiver>&, int device), CudaEngine2D.cu:127-142.
+   receiver_index: padded storage index of each probe (Point::index(r.x+3, r.z+3)). */
+int swl_engine2d_create(const swl_mesh2d_desc *mesh, const int64_t *receiver_index, int receivers, int device,
+                        swl_engine2d **out);
+void swl_engine2d_destroy(swl_engine2d *e);
+/* CudaEngine::step(time, dt, frequency, scale, stepNumber), CudaEngine2D.cu:143-152. */
+int swl_engine2d_step(swl_engine2d *e, double time, double dt, double frequency, double scale, int step);
+/* CudaEngine::rollback / reset, CudaEngine2D.cu:153-154. */
+int swl_engine2d_rollback(swl_engine2d *e);
+int swl_engine2d_reset(swl_engine2d *e);
+/* CudaEngine::diagnostics, CudaEngine2D.cu:155-158. */
+int swl_engine2d_diagnostics(swl_engine2d *e, swl_diag2d *out);
+/* CudaEngine::sample, CudaEngine2D.cu:159-162: out[2r] = Vx, out[2r+1] = Vz. */
+int swl_engine2d_sample(swl_engine2d *e, double *out);
+/* CudaEngine::frame, CudaEngine2D.cu:163-165: component 0 Vz, 1 Vx, 2 |V|, 3 pressure. */
+int swl_engine2d_frame(swl_engine2d *e, int component, float *out, int nx, int nz, int offset_x);
+/* CudaEngine::snapshot, CudaEngine2D.cu:174-178: field f (0..field_count-1) of the accepted
+   state in padded storage promoted to double; swl_engine2d_pml: compact CFS-PML memories. */
+int swl_engine2d_snapshot(swl_engine2d *e, int field, double *out);
+int swl_engine2d_pml(swl_engine2d *e, double *out);
+/* CudaEngine::probes, CudaEngine2D.cu:166-173: out[5n+f] for f = Vx,Vz,Sxx,Szz,Sxz. */
+int swl_engine2d_probes(swl_engine2d *e, const int64_t *indices, int count, double *out);
+/* CudaEngine::residentBytes, CudaEngine2D.cu:179-183. */
+uint64_t swl_engine2d_resident_bytes(swl_engine2d *e);
+
+/* Device-resident stepping (no reference counterpart; same semantics as the reference's
+   advance() loop, CurvilinearSolver2D.cpp:553-597, with no per-step host synchronisation):
+   runs up to `count` steps from `first_step`, each followed on the device by the energy /
+   finiteness inspection, the fault rule
+       !finite || ((s+1)*dt > 3/frequency && peak > 0 && E > 1e4*peak)
+   and the receiver gather.  `peak` is the accepted peak energy before the call.
+   On a rejected step the state stays at the last accepted step and later steps do nothing.
+   Outputs: accepted (steps accepted), faulted (0/1), per accepted step s:
+   diag[4s..4s+3] = energy, max_velocity, traction_relative, finite and
+   samples[(2*receivers)*s + 2r + {0,1}] = Vx, Vz of receiver r.  Arrays hold `count` rows. */
+typedef struct swl_run2d {
+    int accepted, faulted;
+    double peak_energy;
+} swl_run2d;
+int swl_engine2d_run(swl_engine2d *e, int first_step, int count, double dt, double frequency, double scale,
+                     double peak_energy, double *diag, double *samples, swl_run2d *result);
+
+/* ---------------- 2D session (Seis2D::Curvilinear::Simulation) ---------------- */
+typedef struct swl_sim2d swl_sim2d;
+/* backend: 0 CPU is not provided by this library (the CPU path is the reference's own);
+   2 = HIP.  Returns an error for unknown backends instead of falling back. */
+int swl_sim2d_create(const swl_params2d *p, int backend, int device, swl_sim2d **out);
+/* Builds the FP64 session model (validation, geometry, materials, absorbers, sources, dt plan,
+   receivers) without a device: model/diagnostic/layout queries work, stepping is refused. */
+int swl_sim2d_plan(const swl_params2d *p, swl_sim2d **out);
+void swl_sim2d_destroy(swl_sim2d *s);
+int swl_sim2d_advance(swl_sim2d *s, int count);
+int swl_sim2d_reset(swl_sim2d *s);
+int swl_sim2d_step(const swl_sim2d *s);
+int swl_sim2d_total_steps(const swl_sim2d *s);
+double swl_sim2d_dt(const swl_sim2d *s);
+double swl_sim2d_record_dt(const swl_sim2d *s);
+/* layout: nx_pad, nz_pad, offset_x, cpml_width_x, cpml_width_z, field_count, cpml_count,
+   receivers, record_length, mechanisms */
+int swl_sim2d_layout(const swl_sim2d *s, int64_t *out);
+/* diagnostics in the order energy, peak, max_velocity, traction, finite, wave_dt, damping_dt,
+   source_dt, cpml_dt, relaxation_dt, ppw, max_q_error, max_metric, min_jacobian, max_edge,
+   flatten_depth, source_weight_sum, source_rms_radius, initialization_ms, total_step_ms,
+   last_step_ms, resident_device_bytes */
+int swl_sim2d_diagnostics(const swl_sim2d *s, double *out);
+/* receiver records (record_length samples each, t = 0 included) */
+int swl_sim2d_receivers(const swl_sim2d *s, double *vx, double *vz, double *position, int32_t *xz);
+int swl_sim2d_probe_indices(const swl_sim2d *s, int64_t *out);
+int swl_sim2d_snapshot(const swl_sim2d *s, int field, double *out);
+int swl_sim2d_pml(const swl_sim2d *s, double *out);
+/* FP64 construction arrays, same numbering as the oracle (0 a .. 21 deltaS[2]). */
+int swl_sim2d_model(const swl_sim2d *s, int which, double *out);
+int swl_sim2d_read_frame(const swl_sim2d *s, int component, float *out, int include_absorber);
+int swl_sim2d_read_probes(const swl_sim2d *s, const int32_t *xz, int count, double *out);
+/* writeReceivers (CurvilinearSolver2D.cpp:811-854): receiver CSV with provenance header. */
+int swl_sim2d_write_receivers(const swl_sim2d *s, const char *path);
+int64_t swl_pml_slot2d(int nx_pad, int nz_pad, int width_x, int width_z, int64_t q, int axis);
+size_t swl_params2d_size(void);
+size_t swl_params3d_size(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

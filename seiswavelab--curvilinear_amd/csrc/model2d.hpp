@@ -1,0 +1,64 @@
+// model2d.hpp — FP64 construction of a 2D curvilinear session (host side).
+// Restates Seis2D::Curvilinear::Simulation::initialize (src/two_d/CurvilinearSolver2D.cpp:151-540)
+// with the reference's evaluation order, so dt, step counts, index maps and every model array
+// agree bit for bit with the reference's CPU path.
+#pragma once
+#include "swl.h"
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace swl {
+
+struct GslsFit { // SeisWave::Rheology fit restricted to one (Qp, Qs) pair, src/common/Gsls.cpp:11-78
+    int mechanisms = 0;
+    std::array<double, 3> rates{}, strengthP{}, strengthS{};
+};
+struct GslsMaterial {
+    double pRelaxed = 0, sRelaxed = 0, pUnrelaxed = 0, sUnrelaxed = 0, maxRelativeQError = 0;
+    std::array<double, 3> deltaP{}, deltaS{};
+};
+GslsFit gslsFit(int mechanisms, double fmin, double fmax, double qp, double qs);
+GslsMaterial gslsMaterial(const GslsFit &fit, double fref, double fmin, double fmax, double vp, double vs, double rho,
+                          double qp, double qs, bool qError);
+
+// PCHIP terrain with linear continuation (src/two_d/SurfaceProfile.cpp:8-65).
+class Pchip {
+  public:
+    explicit Pchip(const std::vector<std::array<double, 2>> &points);
+    std::array<double, 2> sample(double x) const;
+
+  private:
+    std::vector<std::array<double, 2>> p_;
+    std::vector<double> m_;
+};
+
+struct Model2D {
+    swl_params2d logical{}, p{};   // p: expanded computational parameters
+    int nx = 0, nz = 0, offsetX = 0, widthX = 0, widthZ = 0, fields = 5, mechanisms = 0;
+    size_t count = 0;
+    int64_t cpmlCount = 0;
+    bool explosive = false;
+    double h = 0, flatten = 0, maxMetric = 0, minJacobian = 0, maxEdge = 0, amplitude = 0;
+    std::vector<double> coordinates, normals, a, b, jac, vp, vs, rho, lambda, mu, source, damping, gradX, gradZ;
+    std::array<std::vector<double>, 3> deltaP, deltaS;
+    std::vector<double> cpmlX, cpmlZ;
+    std::array<double, 3> rates{}, direction{};
+    double dt = 0;
+    int total = 0;
+    double waveDt = 0, dampingDt = 0, sourceDt = 0, cpmlDt = 0, relaxationDt = 0, ppw = 0, maxQError = 0;
+    double sourceWeightSum = 0, sourceRmsRadius = 0;
+    std::vector<int32_t> receiverX, receiverZ;
+    std::vector<double> receiverPosition;
+    std::vector<int64_t> probeIndex;
+    size_t recordReserve = 0;
+    std::vector<double> surfaceSamples; // keeps imported samples alive for `logical`
+};
+
+// Throws std::runtime_error with the reference's messages for invalid input.
+Model2D buildModel2D(const swl_params2d &requested);
+swl_mesh2d_desc meshDescriptor(const Model2D &m);
+double lerpExact(double a, double b, double t); // C++20 std::lerp contract (P0811R3)
+
+} // namespace swl
