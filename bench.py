@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Headline benchmark: 2D elastic P-SV, 5001x2001 nodes at h=2 m (30-node external CFS-PML,
+5061x2031 computational nodes), FP32, on one MI355X.
+
+A "step" is one RK4 time step of the whole grid (four fused stage kernels plus the device-side
+inspection / fault rule / receiver gather), inputs resident in HBM.  ``value`` is computational
+node updates per second (G pt-updates/s).  ``roofline`` divides the algorithmic bytes of one
+stage-kernel launch (400 B per computational node per step for FP32 elastic, i.e. 100 B per node
+per launch) by the average stage-kernel launch time measured with HIP events on the engine's
+stream during the timed steps.  ``cpu_baseline`` times the CPU oracle (a bitwise restatement of
+the reference's CPU/OpenMP path) on a bounded sample of the same model on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+PKG_DIR = ROOT / "seiswavelab--curvilinear_amd"
+HBM_PEAK_GBS = 8000.0
+ALGO_BYTES = {"f32": 400.0, "f64": 800.0}  # per computational node per RK4 step, 2D elastic (5 fields, 5 coefficients)
+
+
+def _package():
+    name = "seiswavelab_curvilinear_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, PKG_DIR / "__init__.py", submodule_search_locations=[str(PKG_DIR)])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def measure(pkg, params, steps, warmup, label):
+    sim = pkg.Simulation()
+    t = time.perf_counter()
+    ok, err = sim.initialize(params, pkg.BackendKind.Hip, 0)
+    if not ok:
+        raise RuntimeError(err)
+    log(f"[bench] {label}: initialized in {time.perf_counter() - t:.1f} s, dt={sim.dt():.6g} s")
+    ok, err = sim.advance(warmup)
+    if not ok:
+        raise RuntimeError(err)
+    sim.profile(True)
+    t0 = time.perf_counter()
+    ok, err = sim.advance(steps)
+    t1 = time.perf_counter()
+    if not ok:
+        raise RuntimeError(err)
+    timing = sim.kernel_timing()
+    L = sim.layout()
+    nodes = (L["nx"] - 6) * (L["nz"] - 6)
+    wall = t1 - t0
+    log(f"[bench] {label}: {steps} steps in {wall:.3f} s ({1e3 * wall / steps:.3f} ms/step)")
+    d = sim.diagnostics()
+    sim.close()
+    return {"wall": wall, "timing": timing, "nodes": nodes, "finite": d["finite"], "energy": d["energy"]}
+
+
+def roofline(res, prec):
+    t = res["timing"]
+    avg_ms = t["stage_ms"] / max(t["stage_launches"], 1)
+    per_launch = ALGO_BYTES[prec] / 4 * res["nodes"]
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_ms": round(avg_ms, 5),
+            "algorithmic_bytes_per_launch": per_launch}
+
+
+def cpu_baseline(pkg, params, steps=10, warmup=2):
+    sys.path.insert(0, str(ROOT / "tests"))
+    from cpu_sessions import CpuSession2D
+
+    threads = min(8, os.cpu_count() or 1)
+    params.cpuThreads = threads
+    t = time.perf_counter()
+    s = CpuSession2D("orc", params)
+    log(f"[bench] cpu oracle: model built in {time.perf_counter() - t:.1f} s; {threads} threads")
+    s.advance(warmup)
+    t0 = time.perf_counter()
+    ok, err = s.advance(steps)
+    wall = time.perf_counter() - t0
+    if not ok:
+        raise RuntimeError(err)
+    L = s.layout()
+    nodes = (L["nx"] - 6) * (L["nz"] - 6)
+    s.close()
+    log(f"[bench] cpu oracle: {steps} steps in {wall:.2f} s")
+    return {"value": round(nodes * steps / wall / 1e9, 5), "unit": "G pt-updates/s", "cores": threads, "kind": "port",
+            "sample": f"same 5061x2031-node FP32 model, {steps} timed steps after {warmup} warm-up steps of the "
+                      f"production advance() (RHS, combine, projection, energy inspection, receivers), "
+                      f"OpenMP rows-parallel as the reference"}
+
+
+def traffic_from_profile(prec):
+    """HBM bytes per stage launch from the committed rocprofv3 PMC summary, if present."""
+    path = ROOT / "profiles" / f"stage2d_config2_{prec}_pmc.json"
+    if not path.exists():
+        return None
+    try:
+        data = json.loads(path.read_text())
+        return data.get("traffic_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--fp64-steps", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--nx", type=int, default=5001)
+    ap.add_argument("--nz", type=int, default=2001)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; shots and domains shard per GPU by design "
+              "(see DESIGN.md), and multi-GPU runs are not part of it.", file=sys.stderr)
+        sys.exit(2)
+    pkg = _package()
+    from seiswavelab_curvilinear_amd import workloads
+
+    t = time.perf_counter()
+    w = workloads.config2(seed=2, precision=pkg.Precision.Float32, nx=args.nx, nz=args.nz,
+                          steps=args.warmup + args.steps + 10)
+    log(f"[bench] config-2 model generated in {time.perf_counter() - t:.1f} s")
+    r32 = measure(pkg, w.params, args.steps, args.warmup, "FP32")
+    roof32 = roofline(r32, "f32")
+    roof32["traffic"] = traffic_from_profile("f32")
+    extra = {}
+    if args.fp64_steps > 0:
+        w.params.precision = pkg.Precision.Float64
+        r64 = measure(pkg, w.params, args.fp64_steps, min(args.warmup, 10), "FP64")
+        roof64 = roofline(r64, "f64")
+        extra["fp64"] = {"value": round(r64["nodes"] * args.fp64_steps / r64["wall"] / 1e9, 4),
+                         "ms_per_step": round(1e3 * r64["wall"] / args.fp64_steps, 4), "roofline": roof64}
+        w.params.precision = pkg.Precision.Float32
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(pkg, w.params)
+    value = r32["nodes"] * args.steps / r32["wall"] / 1e9
+    info = pkg.native.device_query(0)
+    line = {
+        "metric": "Gpt-updates/s & %HBM peak, 2D elastic FP32 vs FP64; batched shots/s, p99",
+        "value": round(value, 4),
+        "unit": "G pt-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * r32["wall"] / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded GRF topography and 5% RMS velocity perturbation, seed 2)",
+        "config": {"workload": "2D elastic P-SV single shot, 5001x2001 nodes h=2 m + 30-node external CFS-PML "
+                               "(5061x2031 computational nodes), 30 Hz explosive source, 1001 surface receivers",
+                   "computational_nodes": r32["nodes"], "device": info.name.decode(), "finite": r32["finite"]},
+        "roofline": roof32,
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -176,6 +176,12 @@ typedef struct swl_run2d {
 } swl_run2d;
 int swl_engine2d_run(swl_engine2d *e, int first_step, int count, double dt, double frequency, double scale,
                      double peak_energy, double *diag, double *samples, swl_run2d *result);
+/* Per-launch timing with HIP events on the engine's stream during swl_engine2d_run (no
+   reference counterpart; the reference times whole steps on the host).  enable resets the
+   counters.  timing: stage-kernel ms, stage launches, finalize ms, finalize launches,
+   tiles per launch, stage-kernel VGPRs. */
+int swl_engine2d_profile(swl_engine2d *e, int enable);
+int swl_engine2d_kernel_timing(swl_engine2d *e, double *out);
 
 /* ---------------- 2D session (Seis2D::Curvilinear::Simulation) ---------------- */
 typedef struct swl_sim2d swl_sim2d;
@@ -205,6 +211,8 @@ int swl_sim2d_receivers(const swl_sim2d *s, double *vx, double *vz, double *posi
 int swl_sim2d_probe_indices(const swl_sim2d *s, int64_t *out);
 int swl_sim2d_snapshot(const swl_sim2d *s, int field, double *out);
 int swl_sim2d_pml(const swl_sim2d *s, double *out);
+int swl_sim2d_profile(swl_sim2d *s, int enable);
+int swl_sim2d_kernel_timing(const swl_sim2d *s, double *out);
 /* FP64 construction arrays, same numbering as the oracle (0 a .. 21 deltaS[2]). */
 int swl_sim2d_model(const swl_sim2d *s, int which, double *out);
 int swl_sim2d_read_frame(const swl_sim2d *s, int component, float *out, int include_absorber);

@@ -528,6 +528,36 @@ template <class R> struct Engine2D : EngineBase {
     int registers = 0;
     size_t local_bytes = 0;
     std::vector<Partial> host_partials;
+    // optional per-launch timing with HIP events on the engine stream
+    bool profiling = false;
+    std::vector<hipEvent_t> events;
+    std::vector<int> eventKind; // 0 stage, 1 finalize (one entry per event pair)
+    size_t eventsUsed = 0;
+    double stageMs = 0, finalMs = 0;
+    long long stageLaunches = 0, finalLaunches = 0;
+
+    void mark(int kind, bool begin) {
+        if (!profiling) return;
+        if (eventsUsed == events.size()) {
+            hipEvent_t e;
+            check(hipEventCreate(&e), "event");
+            events.push_back(e);
+        }
+        check(hipEventRecord(events[eventsUsed++], stream), "event record");
+        if (begin) eventKind.push_back(kind);
+    }
+    void collect() {
+        if (!profiling) return;
+        check(hipStreamSynchronize(stream), "timing");
+        for (size_t j = 0; j + 1 < eventsUsed; j += 2) {
+            float ms = 0;
+            check(hipEventElapsedTime(&ms, events[j], events[j + 1]), "event elapsed");
+            if (eventKind[j / 2] == 0) stageMs += ms, ++stageLaunches;
+            else finalMs += ms, ++finalLaunches;
+        }
+        eventsUsed = 0;
+        eventKind.clear();
+    }
 
     R *S(int b) const { return state[b].as<R>(); }
     R *P(int b) const { return pstate[b].as<R>(); }
@@ -608,6 +638,7 @@ template <class R> struct Engine2D : EngineBase {
         }
     }
     ~Engine2D() override {
+        for (auto e : events) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -658,7 +689,9 @@ template <class R> struct Engine2D : EngineBase {
             a.partials = partials.as<Partial>();
             a.ntx = ntx, a.ntiles = ntiles;
             (void)inspect;
+            mark(0, true);
             launch_stage(direction(step, s, 0), direction(step, s, 2), a);
+            mark(0, false);
         }
         return b2;
     }
@@ -790,7 +823,9 @@ template <class R> struct Engine2D : EngineBase {
             f.diag = diag.as<double>(), f.samples = samples.as<double>();
             f.state = S(n), f.probes = probes.as<long long>(), f.nrec = nrec, f.row = j, f.step = s;
             f.dt = dt, f.frequency = frequency;
+            mark(1, true);
             finalize_kernel<R><<<1, THREADS, 0, stream>>>(f);
+            mark(1, false);
             check(hipGetLastError(), "2D finalize launch");
             c = n;
             states[j + 1] = c;
@@ -801,6 +836,7 @@ template <class R> struct Engine2D : EngineBase {
         check(hipMemcpyAsync(&st, status.ptr, sizeof(int), hipMemcpyDeviceToHost, stream), "2D run readback");
         check(hipMemcpyAsync(&pk, peak.ptr, sizeof(double), hipMemcpyDeviceToHost, stream), "2D run readback");
         check(hipStreamSynchronize(stream), "2D run");
+        collect();
         if (acc_steps) {
             if (diag_out)
                 check(hipMemcpy(diag_out, diag.ptr, (size_t)acc_steps * 4 * sizeof(double), hipMemcpyDeviceToHost),
@@ -875,6 +911,23 @@ uint64_t swl_engine2d_resident_bytes(swl_engine2d *e) {
     uint64_t n = 0;
     e->with([&](auto &x) { n = x.resident(); });
     return n;
+}
+int swl_engine2d_profile(swl_engine2d *e, int enable) {
+    return swl::guarded([&] {
+        e->with([&](auto &x) {
+            x.profiling = enable != 0;
+            x.stageMs = x.finalMs = 0;
+            x.stageLaunches = x.finalLaunches = 0;
+        });
+    });
+}
+int swl_engine2d_kernel_timing(swl_engine2d *e, double *out) {
+    return swl::guarded([&] {
+        e->with([&](auto &x) {
+            out[0] = x.stageMs, out[1] = (double)x.stageLaunches, out[2] = x.finalMs, out[3] = (double)x.finalLaunches;
+            out[4] = (double)x.ntiles, out[5] = (double)x.registers;
+        });
+    });
 }
 int swl_engine2d_run(swl_engine2d *e, int first_step, int count, double dt, double frequency, double scale,
                      double peak_energy, double *diag, double *samples, swl_run2d *result) {

@@ -196,6 +196,14 @@ int swl_sim2d_snapshot(const swl_sim2d *s, int field, double *out) {
     return swl_engine2d_snapshot(s->engine, field, out);
 }
 int swl_sim2d_pml(const swl_sim2d *s, double *out) { return swl_engine2d_pml(s->engine, out); }
+int swl_sim2d_profile(swl_sim2d *s, int enable) {
+    if (!s->engine) return swl::guarded([] { throw std::runtime_error("no engine"); });
+    return swl_engine2d_profile(s->engine, enable);
+}
+int swl_sim2d_kernel_timing(const swl_sim2d *s, double *out) {
+    if (!s->engine) return swl::guarded([] { throw std::runtime_error("no engine"); });
+    return swl_engine2d_kernel_timing(s->engine, out);
+}
 int swl_sim2d_model(const swl_sim2d *s, int which, double *out) {
     return swl::guarded([&] {
         const auto &m = s->model;

@@ -42,6 +42,10 @@ SIGNATURES = {
     "swl_engine2d_resident_bytes": (C.c_uint64, [vp]),
     "swl_engine2d_run": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, vp, vp,
                                    P(Run2D)]),
+    "swl_engine2d_profile": (C.c_int, [vp, C.c_int]),
+    "swl_engine2d_kernel_timing": (C.c_int, [vp, vp]),
+    "swl_sim2d_profile": (C.c_int, [vp, C.c_int]),
+    "swl_sim2d_kernel_timing": (C.c_int, [vp, vp]),
     "swl_sim2d_create": (C.c_int, [vp, C.c_int, C.c_int, P(vp)]),
     "swl_sim2d_plan": (C.c_int, [vp, P(vp)]),
     "swl_sim2d_destroy": (None, [vp]),

@@ -174,6 +174,16 @@ class Simulation:
         native.call("swl_sim2d_pml", self._h, _dp(o))
         return o[: L["cpml"]]
 
+    def profile(self, enable: bool = True):
+        """Per-launch HIP-event timing of the stage and finalize kernels (resets counters)."""
+        native.call("swl_sim2d_profile", self._h, int(enable))
+
+    def kernel_timing(self) -> dict:
+        o = np.zeros(6)
+        native.call("swl_sim2d_kernel_timing", self._h, _dp(o))
+        return {"stage_ms": o[0], "stage_launches": int(o[1]), "finalize_ms": o[2], "finalize_launches": int(o[3]),
+                "tiles": int(o[4]), "vgprs": int(o[5])}
+
     def readFrame(self, component: int, includeAbsorber: bool = False):
         L = self.layout()
         p = self._params

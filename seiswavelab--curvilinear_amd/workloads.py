@@ -1,0 +1,114 @@
+"""Seeded synthetic workloads for the benchmark and serving configurations.
+
+Random fields are Gaussian-filtered white noise from ``numpy.random.default_rng(seed)``
+(PCG64).  "Correlation length" L means the Gaussian covariance C(r) = s^2 exp(-r^2 / L^2); the
+filter kernel is therefore exp(-2 x^2 / L^2) and each field is rescaled to the exact requested
+RMS.  Layering is grid-conformal by eta-depth (k*h), as the reference's layered 2D model is
+(src/two_d/CurvilinearSolver2D.cpp:359-373).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from .params import Absorber, GslsSettings, Layer, Parameters, Precision, SourceKind, SourceMechanism
+
+CONFIG1_LAYERS = [Layer(60, 800, 400, 1800), Layer(140, 1800, 900, 2100), Layer(0, 3000, 1700, 2400)]
+
+
+def grf_1d(n: int, h: float, corr: float, rms: float, rng: np.random.Generator) -> np.ndarray:
+    pad = int(math.ceil(3 * corr / h)) + 1
+    white = rng.standard_normal(n + 2 * pad)
+    x = np.arange(-pad, pad + 1) * h
+    k = np.exp(-2 * x * x / (corr * corr))
+    f = np.convolve(white, k, mode="valid")[:n]
+    f -= f.mean()
+    return f * (rms / np.sqrt(np.mean(f * f)))
+
+
+def grf_2d(nz: int, nx: int, h: float, corr: float, rms: float, rng: np.random.Generator) -> np.ndarray:
+    """Periodic FFT filtering on a padded grid (padding > 3 correlation lengths removes wrap-around)."""
+    pad = int(math.ceil(3 * corr / h)) + 1
+    NZ, NX = nz + 2 * pad, nx + 2 * pad
+    white = rng.standard_normal((NZ, NX)).astype(np.float64)
+    kz = np.fft.fftfreq(NZ, d=h) * 2 * np.pi
+    kx = np.fft.rfftfreq(NX, d=h) * 2 * np.pi
+    # Fourier transform of exp(-2 x^2 / L^2) is proportional to exp(-k^2 L^2 / 8)
+    filt = np.exp(-(kz[:, None] ** 2 + kx[None, :] ** 2) * corr * corr / 8)
+    f = np.fft.irfft2(np.fft.rfft2(white) * filt, s=(NZ, NX))[pad:pad + nz, pad:pad + nx]
+    f -= f.mean()
+    return f * (rms / np.sqrt(np.mean(f * f)))
+
+
+def layered_base(nz: int, h: float, layers):
+    """Per-row (k) Vp, Vs, density of the grid-conformal layer table."""
+    vp, vs, rho = np.zeros(nz), np.zeros(nz), np.zeros(nz)
+    for k in range(nz):
+        depth, boundary = k * h, 0.0
+        for i, l in enumerate(layers):
+            boundary += l.thickness
+            if i + 1 == len(layers) or depth < boundary:
+                vp[k], vs[k], rho[k] = l.vp, l.vs, l.density
+                break
+    return vp, vs, rho
+
+
+def config1(seed: int = 1, precision: Precision = Precision.Float64) -> Parameters:
+    """500x200 at h=2: z_s(x)=10 sin(2 pi x/400)+4 sin(2 pi x/137+phase), 3 layers, 30 Hz vertical
+    force at the surface midpoint, 20-node CFS-PML, CFL 0.5, 2000 steps, Vz/Vx at every surface node."""
+    rng = np.random.default_rng(seed)
+    phase = rng.uniform(0, 2 * np.pi)
+    nx, nz, h = 500, 200, 2.0
+    x = np.arange(nx) * h
+    z = 10 * np.sin(2 * np.pi * x / 400) + 4 * np.sin(2 * np.pi * x / 137 + phase)
+    return Parameters(precision=precision, nx=nx, nz=nz, spacing=h, frequency=30, duration=1.0, cfl=.5,
+                      maxSteps=2000, sourceX=-1, sourceZ=0, source=SourceMechanism(SourceKind.DirectionalForce),
+                      absorber=Absorber.CfsPml, spongeX=20, spongeZ=20, receiverStart=0, receiverCount=nx,
+                      receiverStride=1, receiverDepth=0, layered=True, layers=list(CONFIG1_LAYERS),
+                      importedSurface=True, surfaceSamples=np.stack([x, z], 1))
+
+
+@dataclass
+class Heterogeneous2D:
+    params: Parameters
+    seed: int
+    note: str
+
+
+def config2(seed: int = 2, precision: Precision = Precision.Float32, nx: int = 5001, nz: int = 2001,
+            steps: int = 4000, source_z: int = 5) -> Heterogeneous2D:
+    """5001x2001 at h=2: GRF topography (60 m correlation, 25 m RMS), three layers each perturbed by
+    independent 5 % RMS GRFs (20 m correlation) on Vp and Vs with Vp/Vs >= 1.6, 30 Hz Ricker
+    explosive moment, 30-node external CFS-PML, 1001 surface receivers every 10 m.  The explosive
+    source sits 5 nodes deep (10 m), the shallowest the reference accepts."""
+    rng = np.random.default_rng(seed)
+    h = 2.0
+    topo = grf_1d(nx, h, 60.0, 25.0, rng)
+    x = np.arange(nx) * h
+    bvp, bvs, brho = layered_base(nz, h, CONFIG1_LAYERS)
+    dp = grf_2d(nz, nx, h, 20.0, 0.05, rng)
+    ds = grf_2d(nz, nx, h, 20.0, 0.05, rng)
+    vp = bvp[:, None] * (1 + dp)
+    vs = np.minimum(bvs[:, None] * (1 + ds), vp / 1.6)
+    low = vp / vs < 1.6
+    vs[low] = np.nextafter(vs[low], 0)  # keep Vp/Vs >= 1.6 after rounding
+    rho = np.broadcast_to(brho[:, None], (nz, nx)).copy()
+    p = Parameters(precision=precision, nx=nx, nz=nz, spacing=h, frequency=30, duration=2.0, cfl=.8, maxSteps=steps,
+                   sourceX=-1, sourceZ=source_z, source=SourceMechanism(SourceKind.IsotropicMoment),
+                   momentPerLength=2e6, absorber=Absorber.CfsPml, spongeX=30, spongeZ=30, externalAbsorption=True,
+                   receiverStart=0, receiverCount=min(1001, (nx - 1) // 5 + 1), receiverStride=5, receiverDepth=0,
+                   layered=True, layers=list(CONFIG1_LAYERS), importedSurface=True,
+                   surfaceSamples=np.stack([x, topo], 1), nodeVp=vp, nodeVs=vs, nodeDensity=rho)
+    return Heterogeneous2D(p, seed, "config-2 model: seeded GRF topography and velocities")
+
+
+def computational_nodes(p: Parameters) -> int:
+    wx = min(max(p.spongeX, 1), max(1, p.nx // 3))
+    wz = min(max(p.spongeZ, 1), max(1, p.nz // 3))
+    nx, nz = p.nx, p.nz
+    if p.absorbing and p.externalAbsorption:
+        nx += 2 * wx
+        nz += wz
+    return nx * nz
