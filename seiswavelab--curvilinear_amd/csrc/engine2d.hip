@@ -53,6 +53,8 @@ template <class R> struct StageArgs {
     const int *status;
     Partial *partials;
     int ntx, ntiles;
+    const int *tiles; // tile ids handled by this launch (interior or general list)
+    int nlist;
 };
 
 // ---- tile mapping: contiguous band of tiles per XCD (blocks are dispatched round-robin over 8 XCDs)
@@ -185,39 +187,55 @@ __device__ inline void closure(const DMesh<R> &m, const R *__restrict__ w, size_
 }
 
 // Full RHS of one physical node (PointKernels2D.hpp:121-172).  kout[F], prate[8] (X then Z memories).
-template <class R, int DX, int DZ, int MECH>
+// GENERAL = false is the interior specialisation used for tiles below the surface band (depth >= 4),
+// outside the absorber strips and outside the source support: the surface closure, reduced
+// stencils, traction image, ADE memories, damping and source terms are compiled out there, exactly
+// as they are absent from the reference's arithmetic at those nodes.
+template <class R, int DX, int DZ, int MECH, bool GENERAL>
 __device__ inline void rhs_node(const DMesh<R> &m, const R *__restrict__ w, const R *__restrict__ pin, size_t count, int i,
                                 int k, R forcing, R *kout, R *prate, long long &sx, long long &sz) {
     constexpr int F = MECH ? 6 + 4 * MECH : 5;
     const int depth = k - 3;
     const ptrdiff_t q = i + (ptrdiff_t)m.nx * k, s = m.nx;
     const R h = m.h;
-    sx = pml_slot(m.nx, m.nz, m.wx, m.wz, m.cpml, i, k, 0);
-    sz = pml_slot(m.nx, m.nz, m.wx, m.wz, m.cpml, i, k, 1);
+    sx = sz = -1;
+    if constexpr (GENERAL) {
+        sx = pml_slot(m.nx, m.nz, m.wx, m.wz, m.cpml, i, k, 0);
+        sz = pml_slot(m.nx, m.nz, m.wx, m.wz, m.cpml, i, k, 1);
+    }
     R xi[5], eta[5];
 #pragma unroll
     for (int f = 0; f < 5; ++f) xi[f] = d5<R, DX>(w + (size_t)f * count, q, 1, h);
 #pragma unroll
     for (int f = 2; f < 5; ++f) eta[f] = d5<R, DZ>(w + (size_t)f * count, q, s, h);
     R vx[2], vz[2];
-    if (depth == 0) {
-        R xs[2];
-        for (int v = 0; v < 2; ++v) xs[v] = stretched(m, pin, sx, i - 3, 0, v, xi[v], prate);
-        double sur[2];
-        closure(m, w, count, q, i, xs[0], xs[1], sur);
-        eta[0] = (R)sur[0];
-        eta[1] = (R)sur[1];
+    if constexpr (GENERAL) {
+        if (depth == 0) {
+            R xs[2];
+            for (int v = 0; v < 2; ++v) xs[v] = stretched(m, pin, sx, i - 3, 0, v, xi[v], prate);
+            double sur[2];
+            closure(m, w, count, q, i, xs[0], xs[1], sur);
+            eta[0] = (R)sur[0];
+            eta[1] = (R)sur[1];
+        } else {
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+                const R *fv = w + (size_t)v * count;
+                eta[v] = depth == 1 ? d2<R, DZ>(fv, q, s, h) : depth == 2 ? d3<R, DZ>(fv, q, s, h) : d5<R, DZ>(fv, q, s, h);
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+            vx[v] = stretched(m, pin, sx, i - 3, 0, v, xi[v], prate);
+            vz[v] = stretched(m, pin, sz, k - 3, 1, v, eta[v], prate);
+        }
     } else {
 #pragma unroll
         for (int v = 0; v < 2; ++v) {
-            const R *fv = w + (size_t)v * count;
-            eta[v] = depth == 1 ? d2<R, DZ>(fv, q, s, h) : depth == 2 ? d3<R, DZ>(fv, q, s, h) : d5<R, DZ>(fv, q, s, h);
+            eta[v] = d5<R, DZ>(w + (size_t)v * count, q, s, h);
+            vx[v] = xi[v];
+            vz[v] = eta[v];
         }
-    }
-#pragma unroll
-    for (int v = 0; v < 2; ++v) {
-        vx[v] = stretched(m, pin, sx, i - 3, 0, v, xi[v], prate);
-        vz[v] = stretched(m, pin, sz, k - 3, 1, v, eta[v], prate);
     }
     const R a = m.a[q], b = m.b[q], rho = m.rho[q];
     R dx[5], dz[5];
@@ -227,33 +245,40 @@ __device__ inline void rhs_node(const DMesh<R> &m, const R *__restrict__ w, cons
         dx[f] = ux + a * uz;
         dz[f] = b * uz;
     }
-    const bool image = depth <= (DZ ? 3 : 1);
     R o0 = (dx[2] + dz[4]) / rho, o1 = (dx[4] + dz[3]) / rho;
-    if (image && !m.cpml) {
-        o0 = momentum<R, DX, DZ>(m, w, count, q, 0, depth, -1);
-        o1 = momentum<R, DX, DZ>(m, w, count, q, 1, depth, -1);
-    }
-    if (m.cpml) {
-#pragma unroll
-        for (int v = 0; v < 2; ++v) {
-            const int f = v ? 4 : 2, other = v ? 3 : 4;
-            R x = xi[f] / rho, z = (a * eta[f] + b * eta[other]) / rho;
-            if (image) {
-                x = momentum<R, DX, DZ>(m, w, count, q, v, depth, 0);
-                z = momentum<R, DX, DZ>(m, w, count, q, v, depth, 1);
-            }
-            const R r = stretched(m, pin, sx, i - 3, 0, 2 + v, x, prate) + stretched(m, pin, sz, k - 3, 1, 2 + v, z, prate);
-            if (v) o1 = r;
-            else o0 = r;
+    if constexpr (GENERAL) {
+        const bool image = depth <= (DZ ? 3 : 1);
+        if (image && !m.cpml) {
+            o0 = momentum<R, DX, DZ>(m, w, count, q, 0, depth, -1);
+            o1 = momentum<R, DX, DZ>(m, w, count, q, 1, depth, -1);
         }
-    }
-    if (m.has_grad) {
-        o0 += forcing * m.grad0[q] / rho;
-        o1 += forcing * m.grad1[q] / rho;
-    } else {
-        const R sw = m.src[q];
-        o0 += forcing * (sw * m.dir0) / rho;
-        o1 += forcing * (sw * m.dir2) / rho;
+        if (m.cpml) {
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+                const int f = v ? 4 : 2, other = v ? 3 : 4;
+                R x = xi[f] / rho, z = (a * eta[f] + b * eta[other]) / rho;
+                if (image) {
+                    x = momentum<R, DX, DZ>(m, w, count, q, v, depth, 0);
+                    z = momentum<R, DX, DZ>(m, w, count, q, v, depth, 1);
+                }
+                const R r = stretched(m, pin, sx, i - 3, 0, 2 + v, x, prate) + stretched(m, pin, sz, k - 3, 1, 2 + v, z, prate);
+                if (v) o1 = r;
+                else o0 = r;
+            }
+        }
+        if (m.has_grad) {
+            o0 += forcing * m.grad0[q] / rho;
+            o1 += forcing * m.grad1[q] / rho;
+        } else {
+            const R sw = m.src[q];
+            o0 += forcing * (sw * m.dir0) / rho;
+            o1 += forcing * (sw * m.dir2) / rho;
+        }
+    } else if (m.cpml) {
+        // outside the strips the ADE stretch is the identity, but the reference still splits the
+        // momentum into its xi and eta parts (PointKernels2D.hpp:148-153)
+        o0 = xi[2] / rho + (a * eta[2] + b * eta[4]) / rho;
+        o1 = xi[4] / rho + (a * eta[4] + b * eta[3]) / rho;
     }
     const R lambda = m.lam[q], mu = m.mu[q], div = dx[0] + dz[1];
     kout[0] = o0;
@@ -279,10 +304,12 @@ __device__ inline void rhs_node(const DMesh<R> &m, const R *__restrict__ w, cons
             for (int f = 0; f < 4; ++f) kout[kk + f] = m.rates[l] * (drive[f] - mem[f]);
         }
     }
-    if (m.has_damp) {
-        const R d = m.damp[q];
+    if constexpr (GENERAL) {
+        if (m.has_damp) {
+            const R d = m.damp[q];
 #pragma unroll
-        for (int f = 0; f < F; ++f) kout[f] -= d * w[(size_t)f * count + q];
+            for (int f = 0; f < F; ++f) kout[f] -= d * w[(size_t)f * count + q];
+        }
     }
 }
 
@@ -342,11 +369,12 @@ template <class R> __device__ inline void node_stats(const DMesh<R> &m, int k, p
     }
 }
 
-template <class R, int DX, int DZ, int MECH> __global__ __launch_bounds__(THREADS) void stage_kernel(StageArgs<R> A) {
+template <class R, int DX, int DZ, int MECH, bool GENERAL>
+__global__ __launch_bounds__(THREADS) void stage_kernel(StageArgs<R> A) {
     constexpr int F = MECH ? 6 + 4 * MECH : 5;
     __shared__ Partial sh[THREADS];
     if (*A.status) return;
-    const int t = tile_of_block(blockIdx.x, A.ntiles);
+    const int t = A.tiles[tile_of_block(blockIdx.x, A.nlist)];
     const int tx = t % A.ntx, tz = t / A.ntx;
     const int i = 3 + tx * TX + (threadIdx.x % TX), k = 3 + tz * TZ + (threadIdx.x / TX);
     const DMesh<R> &m = A.m;
@@ -355,7 +383,7 @@ template <class R, int DX, int DZ, int MECH> __global__ __launch_bounds__(THREAD
     if (live) {
         R kv[F], pr[8];
         long long sx, sz;
-        rhs_node<R, DX, DZ, MECH>(m, A.in, A.pin, A.count, i, k, A.forcing, kv, pr, sx, sz);
+        rhs_node<R, DX, DZ, MECH, GENERAL>(m, A.in, A.pin, A.count, i, k, A.forcing, kv, pr, sx, sz);
         const ptrdiff_t q = i + (ptrdiff_t)m.nx * k;
         R nxt[F];
 #pragma unroll
@@ -365,38 +393,42 @@ template <class R, int DX, int DZ, int MECH> __global__ __launch_bounds__(THREAD
             nxt[f] = combine(A.kind, A.base[at], kv[f], A.w, acc);
             if (A.kind != 2) A.acc[at] = acc;
         }
-        if (k == 3) {
-            double sxx = nxt[2], szz = nxt[3], sxz = nxt[4];
-            project(m.snrm + 2 * i, sxx, szz, sxz);
-            nxt[2] = (R)sxx, nxt[3] = (R)szz, nxt[4] = (R)sxz;
+        R pn[8];
+        const long long slots[2] = {sx, sz};
+        if constexpr (GENERAL) {
+            if (k == 3) {
+                double sxx = nxt[2], szz = nxt[3], sxz = nxt[4];
+                project(m.snrm + 2 * i, sxx, szz, sxz);
+                nxt[2] = (R)sxx, nxt[3] = (R)szz, nxt[4] = (R)sxz;
+            }
+#pragma unroll
+            for (int axis = 0; axis < 2; ++axis) {
+                const long long sl = slots[axis];
+                if (sl < 0) continue;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    R acc = A.kind == 0 ? (R)0 : A.pacc[sl + c];
+                    pn[4 * axis + c] = combine(A.kind, A.pbase[sl + c], pr[4 * axis + c], A.w, acc);
+                    if (A.kind != 2) A.pacc[sl + c] = acc;
+                    A.pout[sl + c] = pn[4 * axis + c];
+                }
+            }
         }
 #pragma unroll
         for (int f = 0; f < F; ++f) A.out[(size_t)f * A.count + q] = nxt[f];
-        R pn[8];
-        const long long slots[2] = {sx, sz};
-#pragma unroll
-        for (int axis = 0; axis < 2; ++axis) {
-            const long long sl = slots[axis];
-            if (sl < 0) continue;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                R acc = A.kind == 0 ? (R)0 : A.pacc[sl + c];
-                pn[4 * axis + c] = combine(A.kind, A.pbase[sl + c], pr[4 * axis + c], A.w, acc);
-                if (A.kind != 2) A.pacc[sl + c] = acc;
-                A.pout[sl + c] = pn[4 * axis + c];
-            }
-        }
         if (A.kind == 2) {
 #pragma unroll
             for (int f = 0; f < F; ++f)
                 if (!finite_r(nxt[f])) p.bad = 1;
-            const R big = sizeof(R) == 4 ? (R)3.4028234663852886e38 : (R)1.7976931348623157e308;
+            if constexpr (GENERAL) {
+                const R big = sizeof(R) == 4 ? (R)3.4028234663852886e38 : (R)1.7976931348623157e308;
 #pragma unroll
-            for (int axis = 0; axis < 2; ++axis)
-                if (slots[axis] >= 0)
-                    for (int c = 0; c < 4; ++c)
-                        if (!(pn[4 * axis + c] <= big && pn[4 * axis + c] >= -big)) p.bad = 1;
-            node_stats(m, k, q, nxt, p);
+                for (int axis = 0; axis < 2; ++axis)
+                    if (slots[axis] >= 0)
+                        for (int c = 0; c < 4; ++c)
+                            if (!(pn[4 * axis + c] <= big && pn[4 * axis + c] >= -big)) p.bad = 1;
+            }
+            node_stats(m, GENERAL ? k : 4, q, nxt, p);
         }
     }
     if (A.kind == 2) {
@@ -405,10 +437,28 @@ template <class R, int DX, int DZ, int MECH> __global__ __launch_bounds__(THREAD
     }
 }
 
-// Deterministic reduction of per-tile partials, the reference's fault rule, peak update and
-// receiver gather for step s (CurvilinearSolver2D.cpp:576-593).
+// Fixed-partition reduction of the per-tile partials (deterministic: same order every run).
+__global__ __launch_bounds__(THREADS) void reduce_kernel(const Partial *partials, int ntiles, Partial *level2,
+                                                         const int *status) {
+    __shared__ Partial sh[THREADS];
+    if (*status) return;
+    const int per = (ntiles + gridDim.x - 1) / gridDim.x, begin = blockIdx.x * per, end = min(ntiles, begin + per);
+    Partial p{0, 0, 0, 0, 0};
+    for (int t = begin + threadIdx.x; t < end; t += THREADS) {
+        const Partial b = partials[t];
+        p.energy += b.energy;
+        p.velocity = fmax(p.velocity, b.velocity);
+        p.traction = fmax(p.traction, b.traction);
+        p.bad |= b.bad;
+    }
+    block_reduce<double>(p, sh);
+    if (threadIdx.x == 0) level2[blockIdx.x] = sh[0];
+}
+
+// The reference's fault rule, peak update and receiver gather for step s
+// (CurvilinearSolver2D.cpp:576-593), on the reduced diagnostics.
 template <class R> struct FinalArgs {
-    const Partial *partials;
+    const Partial *partials; // level-2 partials
     int ntiles;
     int *status;      // 0 running, 1 faulted
     int *accepted;    // steps accepted in this run
@@ -450,8 +500,6 @@ template <class R> __global__ __launch_bounds__(THREADS) void finalize_kernel(Fi
     }
     __syncthreads();
     if (verdict) return;
-    const size_t count = (size_t)0; // fields are separated by `count`; Vx at state, Vz at state + count
-    (void)count;
     for (int r = threadIdx.x; r < A.nrec; r += THREADS) {
         double *o = A.samples + (size_t)A.row * 2 * A.nrec + 2 * r;
         o[0] = A.state[A.probes[2 * r]];
@@ -518,9 +566,14 @@ template <class R> struct Engine2D : EngineBase {
     int device = 0, nrec = 0, ntx = 0, ntiles = 0, fields = 5, mech = 0, precision = 0;
     size_t count = 0;
     long long cpml = 0;
-    hipStream_t stream{};
+    hipStream_t stream{}, side{};
     DMesh<R> m{};
     DeviceBuffer mesh_arrays[24];
+    // tiles below the surface band, outside absorber strips and source support take the
+    // interior kernel; the rest take the general kernel on a second stream, concurrently
+    DeviceBuffer tileLists, level2;
+    int nInterior = 0, nGeneral = 0, nLevel2 = 0;
+    hipEvent_t fork[4]{}, join[4]{};
     DeviceBuffer state[3], accbuf, pstate[3], paccbuf, partials, probes, status, accepted, peak, diag, samples, gather,
         display;
     int cur = 0, prev = 0;
@@ -622,6 +675,14 @@ template <class R> struct Engine2D : EngineBase {
         ntx = (d.nx - 6 + TX - 1) / TX;
         ntiles = ntx * ((d.nz - 6 + TZ - 1) / TZ);
         partials.allocate((size_t)ntiles * sizeof(Partial));
+        classify_tiles(d);
+        nLevel2 = std::min(ntiles, 128);
+        level2.allocate((size_t)nLevel2 * sizeof(Partial));
+        check(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), "stream");
+        for (int e = 0; e < 4; ++e) {
+            check(hipEventCreateWithFlags(&fork[e], hipEventDisableTiming), "event");
+            check(hipEventCreateWithFlags(&join[e], hipEventDisableTiming), "event");
+        }
         host_partials.resize(ntiles);
         std::vector<long long> pi(2 * (size_t)std::max(n, 1));
         for (int r = 0; r < n; ++r) pi[2 * r] = rec[r], pi[2 * r + 1] = (long long)count + rec[r];
@@ -632,31 +693,99 @@ template <class R> struct Engine2D : EngineBase {
         gather.allocate((size_t)std::max(n, 1) * 5 * sizeof(double));
         display.allocate((size_t)(d.nx - 6) * (d.nz - 6) * sizeof(float));
         hipFuncAttributes attr{};
-        if (hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&stage_kernel<R, 0, 0, 0>)) == hipSuccess) {
+        if (hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&stage_kernel<R, 0, 0, 0, false>)) == hipSuccess) {
             registers = attr.numRegs;
             local_bytes = attr.localSizeBytes;
         }
     }
     ~Engine2D() override {
         for (auto e : events) (void)hipEventDestroy(e);
+        for (int e = 0; e < 4; ++e) {
+            if (fork[e]) (void)hipEventDestroy(fork[e]);
+            if (join[e]) (void)hipEventDestroy(join[e]);
+        }
+        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
-    template <int DX, int DZ> void launch_dirs(const StageArgs<R> &a) {
-        const dim3 grid(ntiles), block(THREADS);
+    // Tile classification: general = surface band (first tile row: depth 0..3), any overlap with the
+    // absorber strips (where CFS-PML memories or sponge damping live) or with the source support.
+    void classify_tiles(const swl_mesh2d_desc &d) {
+        const int nxc = d.nx - 6, nzc = d.nz - 6;
+        int si0 = d.nx, si1 = -1, sk0 = d.nz, sk1 = -1;
+        const double *srcs[3] = {d.source, d.source_gradient[0], d.source_gradient[1]};
+        for (const double *s : srcs) {
+            if (!s) continue;
+            for (int k = 0; k < d.nz; ++k)
+                for (int i = 0; i < d.nx; ++i)
+                    if (s[(size_t)i + (size_t)d.nx * k] != 0) {
+                        si0 = std::min(si0, i), si1 = std::max(si1, i), sk0 = std::min(sk0, k), sk1 = std::max(sk1, k);
+                    }
+        }
+        bool damp = false;
+        int di0 = d.nx, di1 = -1, dk0 = d.nz; // damping support: columns < di0 or > di1, rows >= dk0
+        if (d.damping) {
+            for (int k = 3; k < d.nz - 3; ++k)
+                for (int i = 3; i < d.nx - 3; ++i)
+                    if (d.damping[(size_t)i + (size_t)d.nx * k] != 0) damp = true;
+        }
+        const int wx = cpml || damp ? std::max(d.cpml_width_x, 0) : 0, wz = cpml || damp ? std::max(d.cpml_width_z, 0) : 0;
+        int swx = wx, swz = wz;
+        if (damp && !cpml) { // sponge: nonzero where min(i, nx-1-i) < w or k >= nz - w (computational indices)
+            swx = std::max(1, std::min(d.cpml_width_x, nxc / 3));
+            swz = std::max(1, std::min(d.cpml_width_z, nzc / 3));
+        }
+        (void)di0, (void)di1, (void)dk0;
+        std::vector<int> interior, general;
+        for (int t = 0; t < ntiles; ++t) {
+            const int tx = t % ntx, tz = t / ntx;
+            const int i0 = tx * TX, i1 = std::min(nxc, i0 + TX) - 1, k0 = tz * TZ, k1 = std::min(nzc, k0 + TZ) - 1;
+            bool g = tz == 0;
+            if ((cpml || damp) && (i0 < swx || i1 >= nxc - swx || k1 >= nzc - swz)) g = true;
+            if (si1 >= 0 && !(i1 + 3 < si0 || i0 + 3 > si1 || k1 + 3 < sk0 || k0 + 3 > sk1)) g = true;
+            (g ? general : interior).push_back(t);
+        }
+        nInterior = (int)interior.size(), nGeneral = (int)general.size();
+        std::vector<int> all(interior);
+        all.insert(all.end(), general.begin(), general.end());
+        tileLists.upload(all.data(), all.size() * sizeof(int));
+    }
+    const int *interiorList() const { return tileLists.as<int>(); }
+    const int *generalList() const { return tileLists.as<int>() + nInterior; }
+
+    template <int DX, int DZ, bool G> void launch_dirs(StageArgs<R> a, hipStream_t s) {
+        a.tiles = G ? generalList() : interiorList();
+        a.nlist = G ? nGeneral : nInterior;
+        const dim3 grid(a.nlist), block(THREADS);
         switch (mech) {
-        case 0: stage_kernel<R, DX, DZ, 0><<<grid, block, 0, stream>>>(a); break;
-        case 1: stage_kernel<R, DX, DZ, 1><<<grid, block, 0, stream>>>(a); break;
-        case 2: stage_kernel<R, DX, DZ, 2><<<grid, block, 0, stream>>>(a); break;
-        default: stage_kernel<R, DX, DZ, 3><<<grid, block, 0, stream>>>(a); break;
+        case 0: stage_kernel<R, DX, DZ, 0, G><<<grid, block, 0, s>>>(a); break;
+        case 1: stage_kernel<R, DX, DZ, 1, G><<<grid, block, 0, s>>>(a); break;
+        case 2: stage_kernel<R, DX, DZ, 2, G><<<grid, block, 0, s>>>(a); break;
+        default: stage_kernel<R, DX, DZ, 3, G><<<grid, block, 0, s>>>(a); break;
         }
     }
-    void launch_stage(int dx, int dz, const StageArgs<R> &a) {
-        if (dx == 0 && dz == 0) launch_dirs<0, 0>(a);
-        else if (dx == 0) launch_dirs<0, 1>(a);
-        else if (dz == 0) launch_dirs<1, 0>(a);
-        else launch_dirs<1, 1>(a);
+    template <bool G> void launch_kind(int dx, int dz, const StageArgs<R> &a, hipStream_t s) {
+        if (dx == 0 && dz == 0) launch_dirs<0, 0, G>(a, s);
+        else if (dx == 0) launch_dirs<0, 1, G>(a, s);
+        else if (dz == 0) launch_dirs<1, 0, G>(a, s);
+        else launch_dirs<1, 1, G>(a, s);
         check(hipGetLastError(), "2D stage launch");
+    }
+    // General tiles on the side stream, interior tiles on the engine stream, joined before the
+    // next stage (both read the same input buffer and write disjoint nodes).
+    void launch_stage(int dx, int dz, const StageArgs<R> &a, int slot) {
+        if (nGeneral) {
+            check(hipEventRecord(fork[slot], stream), "fork");
+            check(hipStreamWaitEvent(side, fork[slot], 0), "fork wait");
+            launch_kind<true>(dx, dz, a, side);
+            check(hipEventRecord(join[slot], side), "join");
+        }
+        if (nInterior) {
+            mark(0, true);
+            launch_kind<false>(dx, dz, a, stream);
+            mark(0, false);
+        }
+        if (nGeneral) check(hipStreamWaitEvent(stream, join[slot], 0), "join wait");
     }
 
     static int direction(int step, int stage, int axis) {
@@ -689,9 +818,7 @@ template <class R> struct Engine2D : EngineBase {
             a.partials = partials.as<Partial>();
             a.ntx = ntx, a.ntiles = ntiles;
             (void)inspect;
-            mark(0, true);
-            launch_stage(direction(step, s, 0), direction(step, s, 2), a);
-            mark(0, false);
+            launch_stage(direction(step, s, 0), direction(step, s, 2), a, s);
         }
         return b2;
     }
@@ -817,13 +944,15 @@ template <class R> struct Engine2D : EngineBase {
         for (int j = 0; j < count_steps; ++j) {
             const int s = first + j;
             const int n = enqueue_step(c, dt, frequency, scale, s, true);
+            mark(1, true);
+            reduce_kernel<<<nLevel2, THREADS, 0, stream>>>(partials.as<Partial>(), ntiles, level2.as<Partial>(),
+                                                          status.as<int>());
             FinalArgs<R> f{};
-            f.partials = partials.as<Partial>(), f.ntiles = ntiles;
+            f.partials = level2.as<Partial>(), f.ntiles = nLevel2;
             f.status = status.as<int>(), f.accepted = accepted.as<int>(), f.peak = peak.as<double>();
             f.diag = diag.as<double>(), f.samples = samples.as<double>();
             f.state = S(n), f.probes = probes.as<long long>(), f.nrec = nrec, f.row = j, f.step = s;
             f.dt = dt, f.frequency = frequency;
-            mark(1, true);
             finalize_kernel<R><<<1, THREADS, 0, stream>>>(f);
             mark(1, false);
             check(hipGetLastError(), "2D finalize launch");
