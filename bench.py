@@ -123,6 +123,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nx", type=int, default=5001)
     ap.add_argument("--nz", type=int, default=2001)
+    ap.add_argument("--probe-bytes", type=int, default=1 << 30,
+                    help="size of the plain copy used as the sustained-bandwidth reference (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; shots and domains shard per GPU by design "
@@ -135,9 +137,17 @@ def main():
     w = workloads.config2(seed=2, precision=pkg.Precision.Float32, nx=args.nx, nz=args.nz,
                           steps=args.warmup + args.steps + 10)
     log(f"[bench] config-2 model generated in {time.perf_counter() - t:.1f} s")
+    copy = None
+    if args.probe_bytes > 0:
+        ms = pkg.native.stream_probe(args.probe_bytes, 10)
+        copy = {"bytes": args.probe_bytes, "ms": round(ms, 5), "GBs": round(2 * args.probe_bytes / (ms * 1e-3) / 1e9, 1)}
+        log(f"[bench] plain copy of {args.probe_bytes} B: {ms:.4f} ms = {copy['GBs']} GB/s (read+write)")
     r32 = measure(pkg, w.params, args.steps, args.warmup, "FP32")
     roof32 = roofline(r32, "f32")
     roof32["traffic"] = traffic_from_profile("f32")
+    if copy:
+        roof32["copy_GBs"] = copy["GBs"]
+        roof32["frac_of_copy"] = round(roof32["achieved"] / copy["GBs"], 4)
     extra = {}
     if args.fp64_steps > 0:
         w.params.precision = pkg.Precision.Float64
@@ -166,7 +176,7 @@ def main():
         "data": "synthetic (seeded GRF topography and 5% RMS velocity perturbation, seed 2)",
         "config": {"workload": "2D elastic P-SV single shot, 5001x2001 nodes h=2 m + 30-node external CFS-PML "
                                "(5061x2031 computational nodes), 30 Hz explosive source, 1001 surface receivers",
-                   "computational_nodes": r32["nodes"], "device": info.name.decode(), "finite": r32["finite"]},
+                   "computational_nodes": r32["nodes"], "device": info.arch.decode(), "finite": r32["finite"]},
         "roofline": roof32,
         "cpu_baseline": cpu,
         "extra": extra,

@@ -182,6 +182,10 @@ int swl_engine2d_run(swl_engine2d *e, int first_step, int count, double dt, doub
    tiles per launch, stage-kernel VGPRs. */
 int swl_engine2d_profile(swl_engine2d *e, int enable);
 int swl_engine2d_kernel_timing(swl_engine2d *e, double *out);
+/* Measurement utility (no reference counterpart): `repeats` plain streaming copies of `bytes`
+   (read `bytes`, write `bytes`) on `device`; ms_per_copy = mean HIP-event time per copy.  Used to
+   calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE and as the sustained-copy bandwidth reference. */
+int swl_stream_probe(int device, uint64_t bytes, int repeats, double *ms_per_copy);
 
 /* ---------------- 2D session (Seis2D::Curvilinear::Simulation) ---------------- */
 typedef struct swl_sim2d swl_sim2d;

@@ -1065,3 +1065,38 @@ int swl_engine2d_run(swl_engine2d *e, int first_step, int count, double dt, doub
     });
 }
 }
+
+// ---------------------------------------------------------------------------------------------
+// Plain streaming copy of a known size: calibrates hardware byte counters and measures the copy
+// bandwidth this device actually sustains (a practical ceiling beside the 8 TB/s nominal peak).
+namespace {
+__global__ __launch_bounds__(256) void stream_copy_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                                          size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+} // namespace
+extern "C" int swl_stream_probe(int device, uint64_t bytes, int repeats, double *ms_per_copy) {
+    return swl::guarded([&] {
+        swl::require(bytes >= 16 && repeats > 0, "invalid probe size");
+        swl::DeviceScope scope(device);
+        const size_t n = bytes / sizeof(float4);
+        swl::DeviceBuffer a, b;
+        a.allocate(n * sizeof(float4));
+        b.allocate(n * sizeof(float4));
+        hipEvent_t e0, e1;
+        swl::check(hipEventCreate(&e0), "event");
+        swl::check(hipEventCreate(&e1), "event");
+        const int grid = 256 * 8 * 4;
+        stream_copy_kernel<<<grid, 256>>>(a.as<float4>(), b.as<float4>(), n);
+        swl::check(hipEventRecord(e0, nullptr), "event");
+        for (int r = 0; r < repeats; ++r) stream_copy_kernel<<<grid, 256>>>(a.as<float4>(), b.as<float4>(), n);
+        swl::check(hipEventRecord(e1, nullptr), "event");
+        swl::check(hipEventSynchronize(e1), "probe");
+        float ms = 0;
+        swl::check(hipEventElapsedTime(&ms, e0, e1), "event");
+        *ms_per_copy = ms / repeats;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    });
+}

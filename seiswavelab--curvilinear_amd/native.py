@@ -44,6 +44,7 @@ SIGNATURES = {
                                    P(Run2D)]),
     "swl_engine2d_profile": (C.c_int, [vp, C.c_int]),
     "swl_engine2d_kernel_timing": (C.c_int, [vp, vp]),
+    "swl_stream_probe": (C.c_int, [C.c_int, C.c_uint64, C.c_int, P(C.c_double)]),
     "swl_sim2d_profile": (C.c_int, [vp, C.c_int]),
     "swl_sim2d_kernel_timing": (C.c_int, [vp, vp]),
     "swl_sim2d_create": (C.c_int, [vp, C.c_int, C.c_int, P(vp)]),
@@ -105,6 +106,13 @@ def call(name, *args):
 
 def device_count() -> int:
     return lib().swl_device_count()
+
+
+def stream_probe(nbytes: int, repeats: int = 20, device: int = 0) -> float:
+    """Mean ms per plain copy of `nbytes` (read + write) on the device."""
+    ms = C.c_double()
+    call("swl_stream_probe", device, int(nbytes), int(repeats), C.byref(ms))
+    return ms.value
 
 
 def device_query(index: int = 0) -> DeviceInfo:
