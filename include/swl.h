@@ -124,6 +124,11 @@ typedef struct swl_mesh2d_desc {
     int64_t cpml_count;
     double rates[3];
     const double *delta_p[3], *delta_s[3];
+    /* Optional separable terrain description (CurvilinearSolver2D.cpp:344-354): surface depth
+       H(x_i) and slope H'(x_i) per padded column, fade F(eta_k) and F'(eta_k) per padded row.
+       When a = -(H' F)/(1 + H F'), b = 1/(1 + H F') and J = 1 + H F' reproduce every stored
+       value bit for bit, the engine recomputes them in FP64 instead of streaming the arrays. */
+    const double *surface_depth, *surface_slope, *fade, *fade_derivative;
 } swl_mesh2d_desc;
 
 typedef struct swl_diag2d {
@@ -181,7 +186,11 @@ int swl_engine2d_run(swl_engine2d *e, int first_step, int count, double dt, doub
    counters.  timing: stage-kernel ms, stage launches, finalize ms, finalize launches,
    tiles per launch, stage-kernel VGPRs. */
 int swl_engine2d_profile(swl_engine2d *e, int enable);
+/* timing: interior stage-kernel ms, launches, finalisation ms, launches, interior tiles, interior
+   VGPRs, interior nodes per launch, border tiles, on-the-fly metrics (0/1). */
 int swl_engine2d_kernel_timing(swl_engine2d *e, double *out);
+/* Engine tuning switches: "nontemporal_stores" (default 1) for the interior kernel's outputs. */
+int swl_engine2d_set_option(swl_engine2d *e, const char *name, int value);
 /* Measurement utility (no reference counterpart): `repeats` plain streaming copies of `bytes`
    (read `bytes`, write `bytes`) on `device`; ms_per_copy = mean HIP-event time per copy.  Used to
    calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE and as the sustained-copy bandwidth reference. */
@@ -217,6 +226,7 @@ int swl_sim2d_snapshot(const swl_sim2d *s, int field, double *out);
 int swl_sim2d_pml(const swl_sim2d *s, double *out);
 int swl_sim2d_profile(swl_sim2d *s, int enable);
 int swl_sim2d_kernel_timing(const swl_sim2d *s, double *out);
+int swl_sim2d_set_option(swl_sim2d *s, const char *name, int value);
 /* FP64 construction arrays, same numbering as the oracle (0 a .. 21 deltaS[2]). */
 int swl_sim2d_model(const swl_sim2d *s, int which, double *out);
 int swl_sim2d_read_frame(const swl_sim2d *s, int component, float *out, int include_absorber);

@@ -1,0 +1,606 @@
+// engine2d_host.inl — host side of the 2D HIP engine (included by engine2d.hip).
+// Owns every device allocation, converts between the reference's padded storage (i + nx*k) and
+// the engine's aligned layout (off + i + ld*k), classifies the grid into the interior rectangle and
+// the general border, and enqueues the stage / inspection kernels on the engine's streams.
+namespace swl {
+namespace {
+
+template <class R> struct Engine2D {
+    static constexpr int NV = Lane<R>::N;
+    int device = 0, nrec = 0, fields = 5, mech = 0, precision = 0;
+    int nx = 0, nz = 0, ld = 0, off = ORIGIN;
+    size_t count = 0; // field stride (elements)
+    long long cpml = 0;
+    hipStream_t stream{}, side{};
+    DMesh<R> m{};
+    DeviceBuffer mesh_arrays[32];
+    DeviceBuffer state[3], accbuf, pstate[3], paccbuf, partials, level2, probes, status, accepted, peak, diag, samples,
+        gather, display, generalTiles;
+    int ntx = 0, ntiles = 0;      // 64x4 tiles over the computational grid
+    int nGeneral = 0;             // tiles with at least one node outside the interior rectangle
+    int rtx = 0, nInterior = 0;   // interior tiles (64*NV x 4 nodes)
+    int nPartials = 0, nLevel2 = 0;
+    hipEvent_t fork[4]{}, join[4]{};
+    int cur = 0, prev = 0, diag_rows = 0, registers = 0, nt = 1;
+    size_t local_bytes = 0;
+    std::vector<Partial> host_partials;
+    // optional per-launch timing with HIP events (interior stage kernel and step finalisation)
+    bool profiling = false;
+    std::vector<hipEvent_t> events;
+    std::vector<int> eventKind;
+    size_t eventsUsed = 0;
+    double stageMs = 0, finalMs = 0;
+    long long stageLaunches = 0, finalLaunches = 0;
+
+    R *S(int b) const { return state[b].as<R>(); }
+    R *P(int b) const { return pstate[b].as<R>(); }
+    size_t idx(int i, int k) const { return (size_t)off + i + (size_t)ld * k; }
+    size_t internal_of(int64_t ref) const { return idx((int)(ref % nx), (int)(ref / nx)); }
+
+    // reference layout (i + nx*k, `comps` values per node) -> engine layout, converted to R
+    template <class T> const T *put_layout(int slot, const double *host, int comps) {
+        if (!host) return nullptr;
+        std::vector<T> v((size_t)comps * count, T(0));
+        for (int k = 0; k < nz; ++k)
+            for (int i = 0; i < nx; ++i)
+                for (int c = 0; c < comps; ++c)
+                    v[(size_t)comps * idx(i, k) + c] = (T)host[(size_t)comps * ((size_t)i + (size_t)nx * k) + c];
+        mesh_arrays[slot].upload(v.data(), v.size() * sizeof(T));
+        return mesh_arrays[slot].as<T>();
+    }
+    template <class T> const T *put_plain(int slot, const double *host, size_t n) {
+        if (!host) return nullptr;
+        std::vector<T> v(n);
+        for (size_t i = 0; i < n; ++i) v[i] = (T)host[i];
+        mesh_arrays[slot].upload(v.data(), n * sizeof(T));
+        return mesh_arrays[slot].as<T>();
+    }
+
+    void mark(int kind, bool begin) {
+        if (!profiling) return;
+        if (eventsUsed == events.size()) {
+            hipEvent_t e;
+            check(hipEventCreate(&e), "event");
+            events.push_back(e);
+        }
+        check(hipEventRecord(events[eventsUsed++], stream), "event record");
+        if (begin) eventKind.push_back(kind);
+    }
+    void collect() {
+        if (!profiling) return;
+        check(hipStreamSynchronize(stream), "timing");
+        for (size_t j = 0; j + 1 < eventsUsed; j += 2) {
+            float ms = 0;
+            check(hipEventElapsedTime(&ms, events[j], events[j + 1]), "event elapsed");
+            if (eventKind[j / 2] == 0) stageMs += ms, ++stageLaunches;
+            else finalMs += ms, ++finalLaunches;
+        }
+        eventsUsed = 0;
+        eventKind.clear();
+    }
+
+    Engine2D(const swl_mesh2d_desc &d, const int64_t *rec, int n, int dev) : device(dev), nrec(n) {
+        DeviceScope scope(device);
+        precision = d.precision;
+        nx = d.nx, nz = d.nz;
+        fields = d.field_count;
+        mech = d.mechanisms;
+        cpml = d.cpml_count;
+        require(d.nx >= 14 && d.nz >= 14, "2D mesh needs at least 8 computational nodes per axis");
+        require(mech >= 0 && mech <= 3 && fields == (mech ? 6 + 4 * mech : 5), "inconsistent GSLS field count");
+        ld = (nx + off + 63) / 64 * 64;
+        count = ((size_t)off + (size_t)ld * nz + 63) / 64 * 64;
+        check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+        check(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), "stream");
+        for (int e = 0; e < 4; ++e) {
+            check(hipEventCreateWithFlags(&fork[e], hipEventDisableTiming), "event");
+            check(hipEventCreateWithFlags(&join[e], hipEventDisableTiming), "event");
+        }
+        m.nx = nx, m.nz = nz, m.ld = ld, m.off = off, m.wx = d.cpml_width_x, m.wz = d.cpml_width_z, m.cpml = cpml;
+        m.fields = fields, m.mech = mech;
+        m.h = (R)d.h;
+        for (int l = 0; l < 3; ++l) m.rates[l] = (R)d.rates[l];
+        m.dir0 = (R)d.source_direction[0], m.dir2 = (R)d.source_direction[2];
+        m.a = put_layout<R>(0, d.a, 1), m.b = put_layout<R>(1, d.b, 1), m.jac = put_layout<R>(2, d.jac, 1);
+        m.lam = put_layout<R>(3, d.lambda, 1), m.mu = put_layout<R>(4, d.mu, 1), m.rho = put_layout<R>(5, d.rho, 1);
+        m.src = put_layout<R>(6, d.source, 1), m.nrm = put_layout<R>(8, d.normals, 2);
+        m.has_damp = 0;
+        if (d.damping)
+            for (size_t q = 0; q < (size_t)nx * nz && !m.has_damp; ++q) m.has_damp = d.damping[q] != 0;
+        m.damp = m.has_damp ? put_layout<R>(7, d.damping, 1) : nullptr;
+        m.has_grad = d.source_gradient[0] != nullptr;
+        if (m.has_grad) m.grad0 = put_layout<R>(9, d.source_gradient[0], 1), m.grad1 = put_layout<R>(10, d.source_gradient[1], 1);
+        if (cpml) {
+            m.cx = put_plain<R>(11, d.cpml_x, 3 * (size_t)(nx - 6));
+            m.cz = put_plain<R>(12, d.cpml_z, 3 * (size_t)(nz - 6));
+        }
+        for (int l = 0; l < mech; ++l)
+            m.dP[l] = put_layout<R>(13 + l, d.delta_p[l], 1), m.dS[l] = put_layout<R>(16 + l, d.delta_s[l], 1);
+        // FP64 surface row (k = 3)
+        std::vector<double> sa(nx), sb(nx), sl(nx), sm(nx), sn(2 * (size_t)nx);
+        for (int i = 0; i < nx; ++i) {
+            const size_t q = (size_t)i + (size_t)nx * 3;
+            sa[i] = d.a[q], sb[i] = d.b[q], sl[i] = d.lambda[q], sm[i] = d.mu[q];
+            sn[2 * i] = d.normals[2 * q], sn[2 * i + 1] = d.normals[2 * q + 1];
+        }
+        m.sa = put_plain<double>(19, sa.data(), sa.size()), m.sb = put_plain<double>(20, sb.data(), sb.size());
+        m.slam = put_plain<double>(21, sl.data(), sl.size()), m.smu = put_plain<double>(22, sm.data(), sm.size());
+        m.snrm = put_plain<double>(23, sn.data(), sn.size());
+        setup_metric(d);
+        setup_regions(d);
+        for (auto &b : state) b.allocate((size_t)fields * count * sizeof(R));
+        accbuf.allocate((size_t)fields * count * sizeof(R));
+        if (cpml) {
+            for (auto &b : pstate) b.allocate((size_t)cpml * sizeof(R));
+            paccbuf.allocate((size_t)cpml * sizeof(R));
+        }
+        host_partials.resize(ntiles);
+        std::vector<long long> pi(2 * (size_t)std::max(n, 1));
+        for (int r = 0; r < n; ++r) {
+            require(rec[r] >= 0 && rec[r] < (int64_t)nx * nz, "receiver index outside storage");
+            pi[2 * r] = (long long)internal_of(rec[r]), pi[2 * r + 1] = (long long)count + pi[2 * r];
+        }
+        probes.upload(pi.data(), pi.size() * sizeof(long long));
+        status.allocate(sizeof(int));
+        accepted.allocate(sizeof(int));
+        peak.allocate(sizeof(double));
+        gather.allocate((size_t)std::max(n, 1) * 5 * sizeof(double));
+        display.allocate((size_t)(nx - 6) * (nz - 6) * sizeof(float));
+        hipFuncAttributes attr{};
+        if (hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&interior_kernel<R, 0, 0, 0>)) == hipSuccess) {
+            registers = attr.numRegs;
+            local_bytes = attr.localSizeBytes;
+        }
+    }
+    ~Engine2D() {
+        for (auto e : events) (void)hipEventDestroy(e);
+        for (int e = 0; e < 4; ++e) {
+            if (fork[e]) (void)hipEventDestroy(fork[e]);
+            if (join[e]) (void)hipEventDestroy(join[e]);
+        }
+        if (side) (void)hipStreamDestroy(side);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    // Separable terrain metrics (CurvilinearSolver2D.cpp:349-354): a = -(H' F)/(1 + H F'),
+    // b = 1/(1 + H F'), J = 1 + H F'.  Used on the device only if they reproduce every stored value.
+    void setup_metric(const swl_mesh2d_desc &d) {
+        m.metric = 0;
+        if (!d.surface_depth || !d.surface_slope || !d.fade || !d.fade_derivative) return;
+        for (int k = 0; k < nz; ++k)
+            for (int i = 0; i < nx; ++i) {
+                const size_t q = (size_t)i + (size_t)nx * k;
+                const double zx = d.surface_slope[i] * d.fade[k], j = 1 + d.surface_depth[i] * d.fade_derivative[k];
+                const double a = -zx / j, b = 1 / j;
+                if (std::memcmp(&a, &d.a[q], 8) || std::memcmp(&b, &d.b[q], 8) || std::memcmp(&j, &d.jac[q], 8)) return;
+            }
+        m.tz_ = put_plain<double>(24, d.surface_depth, nx);
+        m.tslope = put_plain<double>(25, d.surface_slope, nx);
+        m.fade = put_plain<double>(26, d.fade, nz);
+        m.fadeder = put_plain<double>(27, d.fade_derivative, nz);
+        m.metric = 1;
+    }
+
+    // Interior rectangle: depth >= 4, outside every CFS-PML slot, zero sponge damping; x-extent
+    // aligned to the lane width.  The source support box stays inside either kernel.
+    void setup_regions(const swl_mesh2d_desc &d) {
+        const int nxc = nx - 6, nzc = nz - 6;
+        int i0 = 3, i1 = nx - 3, k0 = 7, k1 = nz - 3;
+        if (cpml) {
+            i0 = std::max(i0, 3 + d.cpml_width_x), i1 = std::min(i1, nx - 3 - d.cpml_width_x);
+            k1 = std::min(k1, nz - 3 - d.cpml_width_z);
+        }
+        if (m.has_damp) {
+            // columns damped on the top physical row and rows damped in an undamped column
+            int left = 3, right = nx - 3;
+            while (left < nx - 3 && d.damping[(size_t)left + (size_t)nx * 3] != 0) ++left;
+            while (right > left && d.damping[(size_t)(right - 1) + (size_t)nx * 3] != 0) --right;
+            int bottom = nz - 3;
+            const int probe = (left + right) / 2;
+            while (bottom > k0 && d.damping[(size_t)probe + (size_t)nx * (bottom - 1)] != 0) --bottom;
+            i0 = std::max(i0, left), i1 = std::min(i1, right), k1 = std::min(k1, bottom);
+        }
+        i0 = 3 + (i0 - 3 + NV - 1) / NV * NV;
+        i1 = 3 + (std::max(i1, i0) - 3) / NV * NV;
+        if (i1 <= i0 || k1 <= k0) i0 = i1 = k0 = k1 = 0;
+        // verify: no damping and no PML slot inside the rectangle
+        for (int k = k0; k < k1; ++k)
+            for (int i = i0; i < i1; ++i) {
+                bool bad = m.has_damp && d.damping[(size_t)i + (size_t)nx * k] != 0;
+                if (cpml) {
+                    const int ii = i - 3, kk = k - 3, wx = d.cpml_width_x, wz = d.cpml_width_z;
+                    bad = bad || ii < wx || ii >= nxc - wx || kk >= nzc - wz;
+                }
+                if (bad) {
+                    i0 = i1 = k0 = k1 = 0;
+                    k = k1;
+                    break;
+                }
+            }
+        m.ri0 = i0, m.ri1 = i1, m.rk0 = k0, m.rk1 = k1;
+        // source support box
+        m.si0 = nx, m.si1 = -1, m.sk0 = nz, m.sk1 = -1;
+        const double *srcs[3] = {d.source, d.source_gradient[0], d.source_gradient[1]};
+        for (const double *sarr : srcs) {
+            if (!sarr) continue;
+            for (int k = 0; k < nz; ++k)
+                for (int i = 0; i < nx; ++i)
+                    if (sarr[(size_t)i + (size_t)nx * k] != 0)
+                        m.si0 = std::min(m.si0, i), m.si1 = std::max(m.si1, i), m.sk0 = std::min(m.sk0, k),
+                        m.sk1 = std::max(m.sk1, k);
+        }
+        // general tiles: 64x4 tiles with at least one node outside the rectangle
+        ntx = (nxc + TX - 1) / TX;
+        ntiles = ntx * ((nzc + TZ - 1) / TZ);
+        std::vector<int> general;
+        for (int t = 0; t < ntiles; ++t) {
+            const int ti0 = 3 + (t % ntx) * TX, ti1 = std::min(nx - 3, ti0 + TX), tk0 = 3 + (t / ntx) * TZ,
+                      tk1 = std::min(nz - 3, tk0 + TZ);
+            const bool inside = ti0 >= i0 && ti1 <= i1 && tk0 >= k0 && tk1 <= k1;
+            if (!inside) general.push_back(t);
+        }
+        nGeneral = (int)general.size();
+        generalTiles.upload(general.data(), std::max<size_t>(general.size(), 1) * sizeof(int));
+        rtx = i1 > i0 ? (i1 - i0 + 64 * NV - 1) / (64 * NV) : 0;
+        nInterior = rtx ? rtx * ((k1 - k0 + 3) / 4) : 0;
+        nPartials = nGeneral + nInterior;
+        partials.allocate((size_t)std::max(nPartials, 1) * sizeof(Partial));
+        nLevel2 = std::max(1, std::min(nPartials, 128));
+        level2.allocate((size_t)nLevel2 * sizeof(Partial));
+    }
+
+    template <int DX, int DZ> void launch_general(StageArgs<R> a) {
+        a.tiles = generalTiles.as<int>(), a.nlist = nGeneral, a.ntx = ntx, a.partial_base = 0;
+        switch (mech) {
+        case 0: general_kernel<R, DX, DZ, 0><<<nGeneral, THREADS, 0, side>>>(a); break;
+        case 1: general_kernel<R, DX, DZ, 1><<<nGeneral, THREADS, 0, side>>>(a); break;
+        case 2: general_kernel<R, DX, DZ, 2><<<nGeneral, THREADS, 0, side>>>(a); break;
+        default: general_kernel<R, DX, DZ, 3><<<nGeneral, THREADS, 0, side>>>(a); break;
+        }
+    }
+    template <int DX, int DZ> void launch_interior(StageArgs<R> a) {
+        a.nlist = nInterior, a.ntx = rtx, a.partial_base = nGeneral, a.nt = nt;
+        switch (mech) {
+        case 0: interior_kernel<R, DX, DZ, 0><<<nInterior, THREADS, 0, stream>>>(a); break;
+        case 1: interior_kernel<R, DX, DZ, 1><<<nInterior, THREADS, 0, stream>>>(a); break;
+        case 2: interior_kernel<R, DX, DZ, 2><<<nInterior, THREADS, 0, stream>>>(a); break;
+        default: interior_kernel<R, DX, DZ, 3><<<nInterior, THREADS, 0, stream>>>(a); break;
+        }
+    }
+    template <int DX, int DZ> void launch_both(const StageArgs<R> &a, int slot) {
+        // border tiles on the side stream, interior on the engine stream, joined before the next stage
+        if (nGeneral) {
+            check(hipEventRecord(fork[slot], stream), "fork");
+            check(hipStreamWaitEvent(side, fork[slot], 0), "fork wait");
+            launch_general<DX, DZ>(a);
+            check(hipGetLastError(), "2D border stage launch");
+            check(hipEventRecord(join[slot], side), "join");
+        }
+        if (nInterior) {
+            mark(0, true);
+            launch_interior<DX, DZ>(a);
+            check(hipGetLastError(), "2D interior stage launch");
+            mark(0, false);
+        }
+        if (nGeneral) check(hipStreamWaitEvent(stream, join[slot], 0), "join wait");
+    }
+    void launch_stage(int dx, int dz, const StageArgs<R> &a, int slot) {
+        if (dx == 0 && dz == 0) launch_both<0, 0>(a, slot);
+        else if (dx == 0) launch_both<0, 1>(a, slot);
+        else if (dz == 0) launch_both<1, 0>(a, slot);
+        else launch_both<1, 1>(a, slot);
+    }
+
+    static int direction(int step, int stage, int axis) { // FiniteDifference.hpp:9-13
+        static const int flags[8][3] = {{0, 0, 0}, {1, 1, 0}, {1, 1, 1}, {0, 0, 1},
+                                        {0, 1, 0}, {1, 0, 0}, {1, 0, 1}, {0, 1, 1}};
+        return (flags[step % 8][axis] + stage) % 2;
+    }
+    static double pulse(double t, double f) { // CurvilinearSolver2D.cpp:31-34
+        const double a = 3.14159265358979323846 * f * (t - 1.5 / f);
+        return (1 - 2 * a * a) * std::exp(-a * a);
+    }
+
+    // Enqueue one RK4 step from buffer c; returns the buffer holding the new state.
+    int enqueue_step(int c, double dt, double frequency, double scale, int step) {
+        const int b1 = (c + 1) % 3, b2 = (c + 2) % 3;
+        const int in[4] = {c, b1, b2, b1}, out[4] = {b1, b2, b1, b2};
+        const double times[4] = {step * dt, (step + .5) * dt, (step + .5) * dt, (step + 1) * dt};
+        const R weights[4] = {(R)(dt / 2), (R)(dt / 2), (R)dt, (R)dt};
+        for (int s = 0; s < 4; ++s) {
+            StageArgs<R> a{};
+            a.m = m;
+            a.in = S(in[s]), a.base = S(c), a.out = S(out[s]), a.acc = accbuf.as<R>();
+            if (cpml) a.pin = P(in[s]), a.pbase = P(c), a.pout = P(out[s]), a.pacc = paccbuf.as<R>();
+            a.count = count;
+            a.forcing = (R)(scale * pulse(times[s], frequency));
+            a.w = weights[s];
+            a.kind = s == 0 ? 0 : s == 3 ? 2 : 1;
+            a.status = status.as<int>();
+            a.partials = partials.as<Partial>();
+            launch_stage(direction(step, s, 0), direction(step, s, 2), a, s);
+        }
+        return b2;
+    }
+
+    void step(double, double dt, double frequency, double scale, int number) {
+        DeviceScope scope(device);
+        check(hipMemsetAsync(status.ptr, 0, sizeof(int), stream), "2D status clear");
+        prev = cur;
+        cur = enqueue_step(cur, dt, frequency, scale, number);
+        check(hipStreamSynchronize(stream), "2D step");
+    }
+    void rollback() { cur = prev; }
+    void reset() {
+        DeviceScope scope(device);
+        for (auto &b : state) check(hipMemsetAsync(b.ptr, 0, b.bytes, stream), "2D reset");
+        check(hipMemsetAsync(accbuf.ptr, 0, accbuf.bytes, stream), "2D reset");
+        if (cpml) {
+            for (auto &b : pstate) check(hipMemsetAsync(b.ptr, 0, b.bytes, stream), "2D PML reset");
+            check(hipMemsetAsync(paccbuf.ptr, 0, paccbuf.bytes, stream), "2D PML reset");
+        }
+        check(hipStreamSynchronize(stream), "2D clear completion");
+        cur = prev = 0;
+    }
+    void diagnostics(swl_diag2d &out) {
+        DeviceScope scope(device);
+        DeviceBuffer tmp;
+        tmp.allocate((size_t)ntiles * sizeof(Partial));
+        stats_kernel<R><<<ntiles, THREADS, 0, stream>>>(m, S(cur), cpml ? P(cur) : nullptr, count, tmp.as<Partial>(), ntx,
+                                                          ntiles);
+        check(hipGetLastError(), "2D diagnostics");
+        check(hipMemcpyAsync(host_partials.data(), tmp.ptr, ntiles * sizeof(Partial), hipMemcpyDeviceToHost, stream),
+              "2D diagnostics readback");
+        check(hipStreamSynchronize(stream), "2D diagnostics");
+        out.energy = out.max_velocity = out.traction_relative = 0;
+        out.finite = 1;
+        for (const auto &p : host_partials) {
+            out.energy += p.energy;
+            out.max_velocity = std::max(out.max_velocity, p.velocity);
+            out.traction_relative = std::max(out.traction_relative, p.traction);
+            if (p.bad) out.finite = 0;
+        }
+        if (!std::isfinite(out.energy)) out.finite = 0;
+        out.rhs_registers = registers;
+        out.rhs_local_bytes = local_bytes;
+    }
+    void read_indices(const std::vector<long long> &internal, int nf, double *out) {
+        const int n = (int)internal.size();
+        DeviceBuffer di;
+        di.upload(internal.data(), internal.size() * sizeof(long long));
+        gather.allocate((size_t)std::max(n, 1) * nf * sizeof(double));
+        gather_kernel<R><<<(n + 127) / 128, 128, 0, stream>>>(S(cur), count, di.as<long long>(), n, nf, gather.as<double>());
+        check(hipGetLastError(), "2D gather");
+        check(hipMemcpyAsync(out, gather.ptr, (size_t)n * nf * sizeof(double), hipMemcpyDeviceToHost, stream), "2D gather");
+        check(hipStreamSynchronize(stream), "2D gather");
+    }
+    void sample(double *out) {
+        DeviceScope scope(device);
+        std::vector<long long> pi(2 * (size_t)std::max(nrec, 1));
+        check(hipMemcpy(pi.data(), probes.ptr, pi.size() * sizeof(long long), hipMemcpyDeviceToHost), "probe indices");
+        std::vector<long long> idx_(nrec);
+        for (int r = 0; r < nrec; ++r) idx_[r] = pi[2 * r];
+        if (nrec) read_indices(idx_, 2, out);
+    }
+    void probes_at(const int64_t *indices, int n, double *out) {
+        DeviceScope scope(device);
+        std::vector<long long> v(n);
+        for (int r = 0; r < n; ++r) {
+            require(indices[r] >= 0 && indices[r] < (int64_t)nx * nz, "probe index outside storage");
+            v[r] = (long long)internal_of(indices[r]);
+        }
+        if (n) read_indices(v, 5, out);
+    }
+    void snapshot(int f, double *out) {
+        DeviceScope scope(device);
+        require(f >= 0 && f < fields, "2D snapshot field out of range");
+        std::vector<R> v(count);
+        check(hipMemcpy(v.data(), S(cur) + (size_t)f * count, count * sizeof(R), hipMemcpyDeviceToHost), "2D snapshot");
+        for (int k = 0; k < nz; ++k)
+            for (int i = 0; i < nx; ++i) out[(size_t)i + (size_t)nx * k] = v[idx(i, k)];
+    }
+    void pml(double *out) {
+        DeviceScope scope(device);
+        if (!cpml) return;
+        std::vector<R> v(cpml);
+        check(hipMemcpy(v.data(), P(cur), cpml * sizeof(R), hipMemcpyDeviceToHost), "2D PML snapshot");
+        for (long long q = 0; q < cpml; ++q) out[q] = v[q];
+    }
+    void frame(int component, float *out, int fnx, int fnz, int foff) {
+        DeviceScope scope(device);
+        require(component >= 0 && component < 4, "Invalid 2D display component.");
+        require(fnx >= 1 && fnz >= 1 && foff >= 0 && foff + fnx <= nx - 6 && fnz <= nz - 6, "frame outside the domain");
+        const int n = fnx * fnz;
+        frame_kernel<R><<<(n + 127) / 128, 128, 0, stream>>>(m, S(cur), count, component, display.as<float>(), fnx, fnz, foff);
+        check(hipGetLastError(), "2D display launch");
+        check(hipMemcpyAsync(out, display.ptr, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, stream), "2D display");
+        check(hipStreamSynchronize(stream), "2D display");
+    }
+    uint64_t resident() const {
+        uint64_t b = 0;
+        for (const auto &x : mesh_arrays) b += x.bytes;
+        for (const auto &x : state) b += x.bytes;
+        for (const auto &x : pstate) b += x.bytes;
+        for (const DeviceBuffer *x : {&accbuf, &paccbuf, &partials, &level2, &probes, &status, &accepted, &peak, &diag,
+                                      &samples, &gather, &display, &generalTiles})
+            b += x->bytes;
+        return b;
+    }
+
+    // Device-resident stepping with the reference's inspection / fault rule / sampling per step.
+    void run(int first, int count_steps, double dt, double frequency, double scale, double peak0, double *diag_out,
+             double *samples_out, swl_run2d &res) {
+        DeviceScope scope(device);
+        require(count_steps >= 0, "negative step count");
+        res.accepted = 0, res.faulted = 0, res.peak_energy = peak0;
+        if (!count_steps) return;
+        if (diag_rows < count_steps) {
+            diag.allocate((size_t)count_steps * 4 * sizeof(double));
+            samples.allocate((size_t)count_steps * 2 * std::max(nrec, 1) * sizeof(double));
+            diag_rows = count_steps;
+        }
+        check(hipMemsetAsync(status.ptr, 0, sizeof(int), stream), "2D run status");
+        check(hipMemsetAsync(accepted.ptr, 0, sizeof(int), stream), "2D run counter");
+        check(hipMemcpyAsync(peak.ptr, &peak0, sizeof(double), hipMemcpyHostToDevice, stream), "2D run peak");
+        std::vector<int> states(count_steps + 1);
+        int c = cur;
+        states[0] = c;
+        for (int j = 0; j < count_steps; ++j) {
+            const int s = first + j;
+            const int n = enqueue_step(c, dt, frequency, scale, s);
+            mark(1, true);
+            reduce_kernel<<<nLevel2, THREADS, 0, stream>>>(partials.as<Partial>(), nPartials, level2.as<Partial>(),
+                                                          status.as<int>());
+            FinalArgs<R> f{};
+            f.partials = level2.as<Partial>(), f.n = nLevel2;
+            f.status = status.as<int>(), f.accepted = accepted.as<int>(), f.peak = peak.as<double>();
+            f.diag = diag.as<double>(), f.samples = samples.as<double>();
+            f.state = S(n), f.probes = probes.as<long long>(), f.nrec = nrec, f.row = j, f.step = s;
+            f.dt = dt, f.frequency = frequency;
+            finalize_kernel<R><<<1, THREADS, 0, stream>>>(f);
+            mark(1, false);
+            check(hipGetLastError(), "2D finalize launch");
+            c = n;
+            states[j + 1] = c;
+        }
+        int acc_steps = 0, st = 0;
+        double pk = peak0;
+        check(hipMemcpyAsync(&acc_steps, accepted.ptr, sizeof(int), hipMemcpyDeviceToHost, stream), "2D run readback");
+        check(hipMemcpyAsync(&st, status.ptr, sizeof(int), hipMemcpyDeviceToHost, stream), "2D run readback");
+        check(hipMemcpyAsync(&pk, peak.ptr, sizeof(double), hipMemcpyDeviceToHost, stream), "2D run readback");
+        check(hipStreamSynchronize(stream), "2D run");
+        collect();
+        if (acc_steps) {
+            if (diag_out)
+                check(hipMemcpy(diag_out, diag.ptr, (size_t)acc_steps * 4 * sizeof(double), hipMemcpyDeviceToHost),
+                      "2D run diagnostics");
+            if (samples_out && nrec)
+                check(hipMemcpy(samples_out, samples.ptr, (size_t)acc_steps * 2 * nrec * sizeof(double),
+                                hipMemcpyDeviceToHost),
+                      "2D run samples");
+        }
+        res.accepted = acc_steps;
+        res.faulted = st != 0;
+        res.peak_energy = pk;
+        prev = acc_steps ? states[acc_steps - 1] : cur;
+        cur = states[acc_steps];
+    }
+};
+
+} // namespace
+} // namespace swl
+
+struct swl_engine2d {
+    int precision;
+    std::unique_ptr<swl::Engine2D<double>> e64;
+    std::unique_ptr<swl::Engine2D<float>> e32;
+    template <class F> void with(F &&f) {
+        if (e64) f(*e64);
+        else f(*e32);
+    }
+};
+
+extern "C" {
+int swl_engine2d_create(const swl_mesh2d_desc *mesh, const int64_t *receiver_index, int receivers, int device,
+                        swl_engine2d **out) {
+    return swl::guarded([&] {
+        swl::require(mesh && out && receivers >= 0 && (receivers == 0 || receiver_index), "invalid 2D engine arguments");
+        auto e = std::make_unique<swl_engine2d>();
+        e->precision = mesh->precision;
+        if (mesh->precision == 1) e->e32 = std::make_unique<swl::Engine2D<float>>(*mesh, receiver_index, receivers, device);
+        else e->e64 = std::make_unique<swl::Engine2D<double>>(*mesh, receiver_index, receivers, device);
+        *out = e.release();
+    });
+}
+void swl_engine2d_destroy(swl_engine2d *e) { delete e; }
+int swl_engine2d_step(swl_engine2d *e, double time, double dt, double frequency, double scale, int step) {
+    return swl::guarded([&] { e->with([&](auto &x) { x.step(time, dt, frequency, scale, step); }); });
+}
+int swl_engine2d_rollback(swl_engine2d *e) {
+    return swl::guarded([&] { e->with([&](auto &x) { x.rollback(); }); });
+}
+int swl_engine2d_reset(swl_engine2d *e) {
+    return swl::guarded([&] { e->with([&](auto &x) { x.reset(); }); });
+}
+int swl_engine2d_diagnostics(swl_engine2d *e, swl_diag2d *out) {
+    return swl::guarded([&] { e->with([&](auto &x) { x.diagnostics(*out); }); });
+}
+int swl_engine2d_sample(swl_engine2d *e, double *out) {
+    return swl::guarded([&] { e->with([&](auto &x) { x.sample(out); }); });
+}
+int swl_engine2d_frame(swl_engine2d *e, int component, float *out, int nx, int nz, int offset_x) {
+    return swl::guarded([&] { e->with([&](auto &x) { x.frame(component, out, nx, nz, offset_x); }); });
+}
+int swl_engine2d_snapshot(swl_engine2d *e, int field, double *out) {
+    return swl::guarded([&] { e->with([&](auto &x) { x.snapshot(field, out); }); });
+}
+int swl_engine2d_pml(swl_engine2d *e, double *out) {
+    return swl::guarded([&] { e->with([&](auto &x) { x.pml(out); }); });
+}
+int swl_engine2d_probes(swl_engine2d *e, const int64_t *indices, int count, double *out) {
+    return swl::guarded([&] { e->with([&](auto &x) { x.probes_at(indices, count, out); }); });
+}
+uint64_t swl_engine2d_resident_bytes(swl_engine2d *e) {
+    uint64_t n = 0;
+    e->with([&](auto &x) { n = x.resident(); });
+    return n;
+}
+int swl_engine2d_profile(swl_engine2d *e, int enable) {
+    return swl::guarded([&] {
+        e->with([&](auto &x) {
+            x.profiling = enable != 0;
+            x.stageMs = x.finalMs = 0;
+            x.stageLaunches = x.finalLaunches = 0;
+        });
+    });
+}
+int swl_engine2d_kernel_timing(swl_engine2d *e, double *out) {
+    return swl::guarded([&] {
+        e->with([&](auto &x) {
+            out[0] = x.stageMs, out[1] = (double)x.stageLaunches, out[2] = x.finalMs, out[3] = (double)x.finalLaunches;
+            out[4] = (double)x.nInterior, out[5] = (double)x.registers;
+            out[6] = (double)((x.m.ri1 - x.m.ri0) * (long long)(x.m.rk1 - x.m.rk0)); // nodes per interior launch
+            out[7] = (double)x.nGeneral, out[8] = (double)x.m.metric;
+        });
+    });
+}
+int swl_engine2d_set_option(swl_engine2d *e, const char *name, int value) {
+    return swl::guarded([&] {
+        const std::string key = name ? name : "";
+        e->with([&](auto &x) {
+            if (key == "nontemporal_stores") x.nt = value != 0;
+            else throw std::runtime_error("unknown engine option: " + key);
+        });
+    });
+}
+int swl_engine2d_run(swl_engine2d *e, int first_step, int count, double dt, double frequency, double scale,
+                     double peak_energy, double *diag, double *samples, swl_run2d *result) {
+    return swl::guarded([&] {
+        e->with([&](auto &x) { x.run(first_step, count, dt, frequency, scale, peak_energy, diag, samples, *result); });
+    });
+}
+int swl_stream_probe(int device, uint64_t bytes, int repeats, double *ms_per_copy) {
+    return swl::guarded([&] {
+        swl::require(bytes >= 16 && repeats > 0, "invalid probe size");
+        swl::DeviceScope scope(device);
+        const size_t n = bytes / sizeof(float4);
+        swl::DeviceBuffer a, b;
+        a.allocate(n * sizeof(float4));
+        b.allocate(n * sizeof(float4));
+        hipEvent_t e0, e1;
+        swl::check(hipEventCreate(&e0), "event");
+        swl::check(hipEventCreate(&e1), "event");
+        const unsigned grid = (unsigned)((n + 255) / 256);
+        swl::stream_copy_kernel<<<grid, 256>>>(a.as<float4>(), b.as<float4>(), n);
+        swl::check(hipEventRecord(e0, nullptr), "event");
+        for (int r = 0; r < repeats; ++r) swl::stream_copy_kernel<<<grid, 256>>>(a.as<float4>(), b.as<float4>(), n);
+        swl::check(hipEventRecord(e1, nullptr), "event");
+        swl::check(hipEventSynchronize(e1), "probe");
+        float ms = 0;
+        swl::check(hipEventElapsedTime(&ms, e0, e1), "event");
+        *ms_per_copy = ms / repeats;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    });
+}
+}

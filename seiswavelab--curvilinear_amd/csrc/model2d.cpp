@@ -337,12 +337,15 @@ Model2D buildModel2D(const swl_params2d &requested) {
     s.minJacobian = std::numeric_limits<double>::max();
     double maxVp = 0, minVs = std::numeric_limits<double>::max(), maxSigma = 0;
     // metric / material pass (:339-392)
-    std::vector<double> fade(gnz), fadeDer(gnz);
+    std::vector<double> &fade = s.fade, &fadeDer = s.fadeDerivative;
+    fade.assign(gnz, 0), fadeDer.assign(gnz, 0);
     for (int k = 0; k < gnz; ++k) {
         const double eta = (k - 3) * p.spacing, t = std::clamp(eta / s.flatten, 0.0, 1.0);
         fade[k] = 1 - std::pow(t, 5) * (126 + t * (-420 + t * (540 + t * (-315 + 70 * t))));
         fadeDer[k] = eta > 0 && eta < s.flatten ? -630 * std::pow(t, 4) * std::pow(1 - t, 4) / s.flatten : 0;
     }
+    s.surfaceDepth.resize(gnx), s.surfaceSlope.resize(gnx);
+    for (int i = 0; i < gnx; ++i) s.surfaceDepth[i] = surface[i][0], s.surfaceSlope[i] = surface[i][1];
     for (int k = 0; k < gnz; ++k)
         for (int i = 0; i < gnx; ++i) {
             const size_t q = static_cast<size_t>(i) + static_cast<size_t>(gnx) * k;
@@ -577,6 +580,8 @@ swl_mesh2d_desc meshDescriptor(const Model2D &m) {
     }
     for (int l = 0; l < 3; ++l) d.rates[l] = m.rates[l];
     for (int l = 0; l < m.mechanisms; ++l) d.delta_p[l] = m.deltaP[l].data(), d.delta_s[l] = m.deltaS[l].data();
+    d.surface_depth = m.surfaceDepth.data(), d.surface_slope = m.surfaceSlope.data();
+    d.fade = m.fade.data(), d.fade_derivative = m.fadeDerivative.data();
     return d;
 }
 

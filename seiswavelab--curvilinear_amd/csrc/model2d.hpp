@@ -44,6 +44,7 @@ struct Model2D {
     std::vector<double> coordinates, normals, a, b, jac, vp, vs, rho, lambda, mu, source, damping, gradX, gradZ;
     std::array<std::vector<double>, 3> deltaP, deltaS;
     std::vector<double> cpmlX, cpmlZ;
+    std::vector<double> surfaceDepth, surfaceSlope, fade, fadeDerivative; // separable terrain profiles
     std::array<double, 3> rates{}, direction{};
     double dt = 0;
     int total = 0;

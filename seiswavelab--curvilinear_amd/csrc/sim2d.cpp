@@ -204,6 +204,10 @@ int swl_sim2d_kernel_timing(const swl_sim2d *s, double *out) {
     if (!s->engine) return swl::guarded([] { throw std::runtime_error("no engine"); });
     return swl_engine2d_kernel_timing(s->engine, out);
 }
+int swl_sim2d_set_option(swl_sim2d *s, const char *name, int value) {
+    if (!s->engine) return swl::guarded([] { throw std::runtime_error("no engine"); });
+    return swl_engine2d_set_option(s->engine, name, value);
+}
 int swl_sim2d_model(const swl_sim2d *s, int which, double *out) {
     return swl::guarded([&] {
         const auto &m = s->model;

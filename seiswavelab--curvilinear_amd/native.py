@@ -44,9 +44,11 @@ SIGNATURES = {
                                    P(Run2D)]),
     "swl_engine2d_profile": (C.c_int, [vp, C.c_int]),
     "swl_engine2d_kernel_timing": (C.c_int, [vp, vp]),
+    "swl_engine2d_set_option": (C.c_int, [vp, C.c_char_p, C.c_int]),
     "swl_stream_probe": (C.c_int, [C.c_int, C.c_uint64, C.c_int, P(C.c_double)]),
     "swl_sim2d_profile": (C.c_int, [vp, C.c_int]),
     "swl_sim2d_kernel_timing": (C.c_int, [vp, vp]),
+    "swl_sim2d_set_option": (C.c_int, [vp, C.c_char_p, C.c_int]),
     "swl_sim2d_create": (C.c_int, [vp, C.c_int, C.c_int, P(vp)]),
     "swl_sim2d_plan": (C.c_int, [vp, P(vp)]),
     "swl_sim2d_destroy": (None, [vp]),
