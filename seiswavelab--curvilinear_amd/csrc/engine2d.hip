@@ -459,16 +459,22 @@ template <class R, int DX, int DZ, int MECH> __global__ __launch_bounds__(THREAD
 #pragma unroll
         for (int f = 0; f < 5; ++f) {
             const R *fp = A.in + (size_t)f * cnt;
-            // x: the lane's NV nodes plus the neighbouring vectors on each side
-            R row[3 * NV];
-            const V left = vload(fp + q - NV), mid = vload(fp + q), right = vload(fp + q + NV);
+            // x: the lane's NV nodes plus HV vectors on each side (the biased stencil reaches 3 nodes)
+            constexpr int HV = (3 + NV - 1) / NV;
+            R row[(2 * HV + 1) * NV];
+            V mid;
 #pragma unroll
-            for (int j = 0; j < NV; ++j) row[j] = left[j], row[NV + j] = mid[j], row[2 * NV + j] = right[j];
+            for (int v = 0; v <= 2 * HV; ++v) {
+                const V x = vload(fp + q + (v - HV) * NV);
+                if (v == HV) mid = x;
+#pragma unroll
+                for (int j = 0; j < NV; ++j) row[v * NV + j] = x[j];
+            }
 #pragma unroll
             for (int j = 0; j < NV; ++j) {
                 R r = 0;
 #pragma unroll
-                for (int l = 0; l < 5; ++l) r += coef<R, DX>(l) * row[NV + j + offs<DX>(l)];
+                for (int l = 0; l < 5; ++l) r += coef<R, DX>(l) * row[HV * NV + j + offs<DX>(l)];
                 xi[f][j] = r / h;
             }
             // z: five rows of the same columns
