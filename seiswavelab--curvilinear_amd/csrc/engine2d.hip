@@ -51,7 +51,7 @@ template <class R> struct DMesh {
     int nx, nz, ld, off, wx, wz;
     long long cpml;
     int fields, mech, has_damp, has_grad, metric;
-    R h, rates[3], dir0, dir2;
+    R h, hinv, rates[3], dir0, dir2; // hinv = 1/h when h is a power of two (exact), else 0
     const R *a, *b, *jac, *lam, *mu, *rho, *src, *damp, *grad0, *grad1, *nrm, *cx, *cz;
     const R *dP[3], *dS[3];
     const double *sa, *sb, *slam, *smu, *snrm;   // FP64 surface row k = 3, indexed by i
@@ -81,11 +81,10 @@ template <class R> struct StageArgs {
     R forcing, w;
     int kind; // 0 first stage, 1 middle stages, 2 final stage (+ inspection)
     const int *status;
-    Partial *partials; // one per tile of this launch, at partial_base + tile index
-    int partial_base;
-    const int *tiles; // general: list of 64x4 tile ids; interior: unused
-    int nlist, ntx;   // general: list length and tiles per row; interior: tiles, tiles per row
-    int nt;           // interior: nontemporal stores
+    Partial *partials; // border tiles first, then interior tiles
+    const int *tiles;  // border tile ids (64x4 tiles, gntx per row)
+    int ngeneral, gblocks, gntx, ninterior, rtx;
+    int nt; // interior: nontemporal stores
 };
 
 // ---- tile mapping: contiguous band of tiles per XCD (blocks are dispatched round-robin over 8 XCDs)
@@ -100,14 +99,16 @@ template <class R, int DIR> __device__ inline R coef(int l) {
     return DIR ? -c : c;
 }
 template <int DIR> __device__ constexpr int offs(int l) { return DIR ? -(l - 1) : (l - 1); }
+// r / h, computed as r * (1/h) when h is a power of two (the two are then the same IEEE result)
+template <class R> __device__ inline R over_h(R r, R h, R hinv) { return hinv != (R)0 ? r * hinv : r / h; }
 
-template <class R, int DIR> __device__ inline R d5(const R *__restrict__ v, ptrdiff_t q, ptrdiff_t s, R h) {
+template <class R, int DIR> __device__ inline R d5(const R *__restrict__ v, ptrdiff_t q, ptrdiff_t s, R h, R hinv) {
     R r = 0;
 #pragma unroll
     for (int l = 0; l < 5; ++l) r += coef<R, DIR>(l) * v[q + offs<DIR>(l) * s];
-    return r / h;
+    return over_h(r, h, hinv);
 }
-template <class R, int DIR> __device__ inline R d3(const R *__restrict__ v, ptrdiff_t q, ptrdiff_t s, R h) {
+template <class R, int DIR> __device__ inline R d3(const R *__restrict__ v, ptrdiff_t q, ptrdiff_t s, R h, R hinv) {
     const R l0 = ((R)-7) / ((R)6), l1 = ((R)8) / ((R)6), l2 = ((R)-1) / ((R)6);
     const R c0 = DIR ? -l0 : l0, c1 = DIR ? -l1 : l1, c2 = DIR ? -l2 : l2;
     constexpr int o1 = DIR ? -1 : 1, o2 = DIR ? -2 : 2;
@@ -115,15 +116,15 @@ template <class R, int DIR> __device__ inline R d3(const R *__restrict__ v, ptrd
     r += c0 * v[q];
     r += c1 * v[q + o1 * s];
     r += c2 * v[q + o2 * s];
-    return r / h;
+    return over_h(r, h, hinv);
 }
-template <class R, int DIR> __device__ inline R d2(const R *__restrict__ v, ptrdiff_t q, ptrdiff_t s, R h) {
+template <class R, int DIR> __device__ inline R d2(const R *__restrict__ v, ptrdiff_t q, ptrdiff_t s, R h, R hinv) {
     const R c0 = DIR ? (R)1 : (R)-1, c1 = DIR ? (R)-1 : (R)1;
     constexpr int o1 = DIR ? -1 : 1;
     R r = 0;
     r += c0 * v[q];
     r += c1 * v[q + o1 * s];
-    return r / h;
+    return over_h(r, h, hinv);
 }
 
 // ---- compact CFS-PML slots (PointKernels2D.hpp:50-56)
@@ -216,9 +217,9 @@ __device__ inline void rhs_general(const DMesh<R> &m, const R *__restrict__ w, c
     sz = pml_slot(m.nx, m.nz, m.wx, m.wz, m.cpml, i, k, 1);
     R xi[5], eta[5];
 #pragma unroll
-    for (int f = 0; f < 5; ++f) xi[f] = d5<R, DX>(w + (size_t)f * count, q, 1, h);
+    for (int f = 0; f < 5; ++f) xi[f] = d5<R, DX>(w + (size_t)f * count, q, 1, h, m.hinv);
 #pragma unroll
-    for (int f = 2; f < 5; ++f) eta[f] = d5<R, DZ>(w + (size_t)f * count, q, s, h);
+    for (int f = 2; f < 5; ++f) eta[f] = d5<R, DZ>(w + (size_t)f * count, q, s, h, m.hinv);
     R vx[2], vz[2];
     if (depth == 0) {
         R xs[2];
@@ -231,7 +232,9 @@ __device__ inline void rhs_general(const DMesh<R> &m, const R *__restrict__ w, c
 #pragma unroll
         for (int v = 0; v < 2; ++v) {
             const R *fv = w + (size_t)v * count;
-            eta[v] = depth == 1 ? d2<R, DZ>(fv, q, s, h) : depth == 2 ? d3<R, DZ>(fv, q, s, h) : d5<R, DZ>(fv, q, s, h);
+            eta[v] = depth == 1   ? d2<R, DZ>(fv, q, s, h, m.hinv)
+                     : depth == 2 ? d3<R, DZ>(fv, q, s, h, m.hinv)
+                                  : d5<R, DZ>(fv, q, s, h, m.hinv);
         }
     }
 #pragma unroll
@@ -364,12 +367,11 @@ __device__ inline void node_stats(R rho, R lam, R mu, R jac, R h, const R *u, bo
 
 // ------------------------------------------------------------------------------------------------
 // General kernel: 64x4 tiles from a list; nodes inside the interior rectangle are skipped.
-template <class R, int DX, int DZ, int MECH> __global__ __launch_bounds__(THREADS) void general_kernel(StageArgs<R> A) {
+template <class R, int DX, int DZ, int MECH>
+__device__ __forceinline__ void general_body(const StageArgs<R> &A, int li, Partial *sh) {
     constexpr int F = MECH ? 6 + 4 * MECH : 5;
-    __shared__ Partial sh[THREADS];
-    if (*A.status) return;
-    const int li = tile_of_block(blockIdx.x, A.nlist), t = A.tiles[li];
-    const int i = 3 + (t % A.ntx) * TX + (threadIdx.x % TX), k = 3 + (t / A.ntx) * TZ + (threadIdx.x / TX);
+    const int t = A.tiles[li];
+    const int i = 3 + (t % A.gntx) * TX + (threadIdx.x % TX), k = 3 + (t / A.gntx) * TZ + (threadIdx.x / TX);
     const DMesh<R> &m = A.m;
     const bool live = i < m.nx - 3 && k < m.nz - 3 && !in_rect(m, i, k);
     Partial p{0, 0, 0, 0, 0};
@@ -423,7 +425,7 @@ template <class R, int DX, int DZ, int MECH> __global__ __launch_bounds__(THREAD
     }
     if (A.kind == 2) {
         block_reduce(p, sh);
-        if (threadIdx.x == 0) A.partials[A.partial_base + li] = sh[0];
+        if (threadIdx.x == 0) A.partials[li] = sh[0];
     }
 }
 
@@ -440,15 +442,13 @@ template <class R> __device__ inline void vstore(R *p, typename Lane<R>::V v, in
     else *d = v;
 }
 
-template <class R, int DX, int DZ, int MECH> __global__ __launch_bounds__(THREADS) void interior_kernel(StageArgs<R> A) {
+template <class R, int DX, int DZ, int MECH>
+__device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Partial *sh) {
     using V = typename Lane<R>::V;
     constexpr int NV = Lane<R>::N;
     constexpr int F = MECH ? 6 + 4 * MECH : 5;
-    __shared__ Partial sh[THREADS];
-    if (*A.status) return;
     const DMesh<R> &m = A.m;
-    const int tl = tile_of_block(blockIdx.x, A.nlist);
-    const int i0 = m.ri0 + ((tl % A.ntx) * 64 + (threadIdx.x % 64)) * NV, k = m.rk0 + (tl / A.ntx) * 4 + threadIdx.x / 64;
+    const int i0 = m.ri0 + ((tl % A.rtx) * 64 + (threadIdx.x % 64)) * NV, k = m.rk0 + (tl / A.rtx) * 4 + threadIdx.x / 64;
     const bool live = i0 < m.ri1 && k < m.rk1;
     Partial p{0, 0, 0, 0, 0};
     if (live) {
@@ -456,6 +456,7 @@ template <class R, int DX, int DZ, int MECH> __global__ __launch_bounds__(THREAD
         const size_t cnt = A.count;
         const R h = m.h;
         R xi[5][NV], eta[5][NV];
+        V centre[5];
 #pragma unroll
         for (int f = 0; f < 5; ++f) {
             const R *fp = A.in + (size_t)f * cnt;
@@ -466,7 +467,7 @@ template <class R, int DX, int DZ, int MECH> __global__ __launch_bounds__(THREAD
 #pragma unroll
             for (int v = 0; v <= 2 * HV; ++v) {
                 const V x = vload(fp + q + (v - HV) * NV);
-                if (v == HV) mid = x;
+                if (v == HV) mid = x, centre[f] = x;
 #pragma unroll
                 for (int j = 0; j < NV; ++j) row[v * NV + j] = x[j];
             }
@@ -475,7 +476,7 @@ template <class R, int DX, int DZ, int MECH> __global__ __launch_bounds__(THREAD
                 R r = 0;
 #pragma unroll
                 for (int l = 0; l < 5; ++l) r += coef<R, DX>(l) * row[HV * NV + j + offs<DX>(l)];
-                xi[f][j] = r / h;
+                xi[f][j] = over_h(r, h, m.hinv);
             }
             // z: five rows of the same columns
             V col[5];
@@ -486,7 +487,7 @@ template <class R, int DX, int DZ, int MECH> __global__ __launch_bounds__(THREAD
                 R r = 0;
 #pragma unroll
                 for (int l = 0; l < 5; ++l) r += coef<R, DZ>(l) * col[l][j];
-                eta[f][j] = r / h;
+                eta[f][j] = over_h(r, h, m.hinv);
             }
         }
         // separable metrics in FP64 (bit-identical to the host arrays) or the stored arrays
@@ -571,7 +572,7 @@ template <class R, int DX, int DZ, int MECH> __global__ __launch_bounds__(THREAD
 #pragma unroll
         for (int f = 0; f < F; ++f) {
             const size_t o = (size_t)f * cnt + q;
-            const V vb = vload(A.base + o);
+            const V vb = A.kind == 0 && f < 5 ? centre[f < 5 ? f : 0] : vload(A.base + o); // stage 1: base is the input
             V va;
             if (A.kind != 0) va = vload(A.acc + o);
             V vo, vacc;
@@ -600,7 +601,21 @@ template <class R, int DX, int DZ, int MECH> __global__ __launch_bounds__(THREAD
     }
     if (A.kind == 2) {
         block_reduce(p, sh);
-        if (threadIdx.x == 0) A.partials[A.partial_base + tl] = sh[0];
+        if (threadIdx.x == 0) A.partials[A.ngeneral + tl] = sh[0];
+    }
+}
+
+// One launch per RK stage: blocks [0, gblocks) take the border tiles (gblocks is ngeneral rounded up to
+// a multiple of the 8 XCDs, so the interior blocks keep their XCD-contiguous band mapping), the rest
+// take the interior tiles.  A single in-order launch avoids cross-stream joins between stages.
+template <class R, int DX, int DZ, int MECH> __global__ __launch_bounds__(THREADS) void stage_kernel(StageArgs<R> A) {
+    __shared__ Partial sh[THREADS];
+    if (*A.status) return;
+    const int b = blockIdx.x;
+    if (b < A.gblocks) {
+        if (b < A.ngeneral) general_body<R, DX, DZ, MECH>(A, b, sh);
+    } else {
+        interior_body<R, DX, DZ, MECH>(A, tile_of_block(b - A.gblocks, A.ninterior), sh);
     }
 }
 

@@ -99,6 +99,11 @@ template <class R> struct Engine2D {
         m.nx = nx, m.nz = nz, m.ld = ld, m.off = off, m.wx = d.cpml_width_x, m.wz = d.cpml_width_z, m.cpml = cpml;
         m.fields = fields, m.mech = mech;
         m.h = (R)d.h;
+        {
+            int e = 0;
+            const double frac = std::frexp(d.h, &e); // h = frac * 2^e, power of two iff frac == 0.5
+            m.hinv = frac == 0.5 ? (R)(1 / d.h) : (R)0;
+        }
         for (int l = 0; l < 3; ++l) m.rates[l] = (R)d.rates[l];
         m.dir0 = (R)d.source_direction[0], m.dir2 = (R)d.source_direction[2];
         m.a = put_layout<R>(0, d.a, 1), m.b = put_layout<R>(1, d.b, 1), m.jac = put_layout<R>(2, d.jac, 1);
@@ -147,7 +152,7 @@ template <class R> struct Engine2D {
         gather.allocate((size_t)std::max(n, 1) * 5 * sizeof(double));
         display.allocate((size_t)(nx - 6) * (nz - 6) * sizeof(float));
         hipFuncAttributes attr{};
-        if (hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&interior_kernel<R, 0, 0, 0>)) == hipSuccess) {
+        if (hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&stage_kernel<R, 0, 0, 0>)) == hipSuccess) {
             registers = attr.numRegs;
             local_bytes = attr.localSizeBytes;
         }
@@ -249,40 +254,19 @@ template <class R> struct Engine2D {
         level2.allocate((size_t)nLevel2 * sizeof(Partial));
     }
 
-    template <int DX, int DZ> void launch_general(StageArgs<R> a) {
-        a.tiles = generalTiles.as<int>(), a.nlist = nGeneral, a.ntx = ntx, a.partial_base = 0;
+    template <int DX, int DZ> void launch_both(StageArgs<R> a, int) {
+        a.tiles = generalTiles.as<int>(), a.ngeneral = nGeneral, a.gblocks = (nGeneral + 7) / 8 * 8, a.gntx = ntx;
+        a.ninterior = nInterior, a.rtx = rtx, a.nt = nt;
+        const int blocks = a.gblocks + nInterior;
+        mark(0, true);
         switch (mech) {
-        case 0: general_kernel<R, DX, DZ, 0><<<nGeneral, THREADS, 0, side>>>(a); break;
-        case 1: general_kernel<R, DX, DZ, 1><<<nGeneral, THREADS, 0, side>>>(a); break;
-        case 2: general_kernel<R, DX, DZ, 2><<<nGeneral, THREADS, 0, side>>>(a); break;
-        default: general_kernel<R, DX, DZ, 3><<<nGeneral, THREADS, 0, side>>>(a); break;
+        case 0: stage_kernel<R, DX, DZ, 0><<<blocks, THREADS, 0, stream>>>(a); break;
+        case 1: stage_kernel<R, DX, DZ, 1><<<blocks, THREADS, 0, stream>>>(a); break;
+        case 2: stage_kernel<R, DX, DZ, 2><<<blocks, THREADS, 0, stream>>>(a); break;
+        default: stage_kernel<R, DX, DZ, 3><<<blocks, THREADS, 0, stream>>>(a); break;
         }
-    }
-    template <int DX, int DZ> void launch_interior(StageArgs<R> a) {
-        a.nlist = nInterior, a.ntx = rtx, a.partial_base = nGeneral, a.nt = nt;
-        switch (mech) {
-        case 0: interior_kernel<R, DX, DZ, 0><<<nInterior, THREADS, 0, stream>>>(a); break;
-        case 1: interior_kernel<R, DX, DZ, 1><<<nInterior, THREADS, 0, stream>>>(a); break;
-        case 2: interior_kernel<R, DX, DZ, 2><<<nInterior, THREADS, 0, stream>>>(a); break;
-        default: interior_kernel<R, DX, DZ, 3><<<nInterior, THREADS, 0, stream>>>(a); break;
-        }
-    }
-    template <int DX, int DZ> void launch_both(const StageArgs<R> &a, int slot) {
-        // border tiles on the side stream, interior on the engine stream, joined before the next stage
-        if (nGeneral) {
-            check(hipEventRecord(fork[slot], stream), "fork");
-            check(hipStreamWaitEvent(side, fork[slot], 0), "fork wait");
-            launch_general<DX, DZ>(a);
-            check(hipGetLastError(), "2D border stage launch");
-            check(hipEventRecord(join[slot], side), "join");
-        }
-        if (nInterior) {
-            mark(0, true);
-            launch_interior<DX, DZ>(a);
-            check(hipGetLastError(), "2D interior stage launch");
-            mark(0, false);
-        }
-        if (nGeneral) check(hipStreamWaitEvent(stream, join[slot], 0), "join wait");
+        check(hipGetLastError(), "2D stage launch");
+        mark(0, false);
     }
     void launch_stage(int dx, int dz, const StageArgs<R> &a, int slot) {
         if (dx == 0 && dz == 0) launch_both<0, 0>(a, slot);
@@ -559,7 +543,7 @@ int swl_engine2d_kernel_timing(swl_engine2d *e, double *out) {
         e->with([&](auto &x) {
             out[0] = x.stageMs, out[1] = (double)x.stageLaunches, out[2] = x.finalMs, out[3] = (double)x.finalLaunches;
             out[4] = (double)x.nInterior, out[5] = (double)x.registers;
-            out[6] = (double)((x.m.ri1 - x.m.ri0) * (long long)(x.m.rk1 - x.m.rk0)); // nodes per interior launch
+            out[6] = (double)((x.nx - 6) * (long long)(x.nz - 6)); // nodes updated per stage launch
             out[7] = (double)x.nGeneral, out[8] = (double)x.m.metric;
         });
     });
