@@ -41,14 +41,15 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def measure(pkg, params, steps, warmup, label, nt=1):
+def measure(pkg, params, steps, warmup, label, options=None):
     sim = pkg.Simulation()
     t = time.perf_counter()
     ok, err = sim.initialize(params, pkg.BackendKind.Hip, 0)
     if not ok:
         raise RuntimeError(err)
     log(f"[bench] {label}: initialized in {time.perf_counter() - t:.1f} s, dt={sim.dt():.6g} s")
-    sim.set_option("nontemporal_stores", nt)
+    for name, value in (options or {}).items():
+        sim.set_option(name, value)
     ok, err = sim.advance(warmup)
     if not ok:
         raise RuntimeError(err)
@@ -71,7 +72,8 @@ def measure(pkg, params, steps, warmup, label, nt=1):
 def roofline(res, prec):
     t = res["timing"]
     avg_ms = t["stage_ms"] / max(t["stage_launches"], 1)
-    # the dominant kernel is the interior stage kernel; one launch updates its rectangle of nodes
+    # the dominant kernel is the fused RK-stage kernel; one launch updates every computational node
+    # (interior and border tiles), and a step is four such launches
     per_launch = ALGO_BYTES[prec] / 4 * t["interior_nodes"]
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -127,6 +129,7 @@ def main():
     ap.add_argument("--nx", type=int, default=5001)
     ap.add_argument("--nz", type=int, default=2001)
     ap.add_argument("--nt", type=int, default=1, help="nontemporal stores in the interior kernel")
+    ap.add_argument("--stored-metrics", type=int, default=0, help="read a, b, J from memory instead of recomputing")
     ap.add_argument("--probe-bytes", type=int, default=1 << 30,
                     help="size of the plain copy used as the sustained-bandwidth reference (0 = skip)")
     args = ap.parse_args()
@@ -146,7 +149,8 @@ def main():
         ms = pkg.native.stream_probe(args.probe_bytes, 10)
         copy = {"bytes": args.probe_bytes, "ms": round(ms, 5), "GBs": round(2 * args.probe_bytes / (ms * 1e-3) / 1e9, 1)}
         log(f"[bench] plain copy of {args.probe_bytes} B: {ms:.4f} ms = {copy['GBs']} GB/s (read+write)")
-    r32 = measure(pkg, w.params, args.steps, args.warmup, "FP32", args.nt)
+    options = {"nontemporal_stores": args.nt, "stored_metrics": args.stored_metrics}
+    r32 = measure(pkg, w.params, args.steps, args.warmup, "FP32", options)
     roof32 = roofline(r32, "f32")
     roof32["traffic"] = traffic_from_profile("f32")
     if copy:
@@ -155,7 +159,7 @@ def main():
     extra = {}
     if args.fp64_steps > 0:
         w.params.precision = pkg.Precision.Float64
-        r64 = measure(pkg, w.params, args.fp64_steps, min(args.warmup, 10), "FP64", args.nt)
+        r64 = measure(pkg, w.params, args.fp64_steps, min(args.warmup, 10), "FP64", options)
         roof64 = roofline(r64, "f64")
         extra["fp64"] = {"value": round(r64["nodes"] * args.fp64_steps / r64["wall"] / 1e9, 4),
                          "ms_per_step": round(1e3 * r64["wall"] / args.fp64_steps, 4), "roofline": roof64}

@@ -51,7 +51,7 @@ template <class R> struct DMesh {
     int nx, nz, ld, off, wx, wz;
     long long cpml;
     int fields, mech, has_damp, has_grad, metric;
-    R h, hinv, rates[3], dir0, dir2; // hinv = 1/h when h is a power of two (exact), else 0
+    R h, hinv, rates[3], dir0, dir2; // hinv = 1/h (FP64: only when h is a power of two, so exact), else 0
     const R *a, *b, *jac, *lam, *mu, *rho, *src, *damp, *grad0, *grad1, *nrm, *cx, *cz;
     const R *dP[3], *dS[3];
     const double *sa, *sb, *slam, *smu, *snrm;   // FP64 surface row k = 3, indexed by i
@@ -78,8 +78,8 @@ template <class R> struct StageArgs {
     const R *pin, *pbase;
     R *pout, *pacc;
     size_t count;
-    R forcing, w;
-    int kind; // 0 first stage, 1 middle stages, 2 final stage (+ inspection)
+    R forcing, w, wy; // wy: weight of this stage's k in Y = U^n + dt(k1 + 2k2 + 2k3 + k4)/6 (FP32 interior)
+    int kind;         // 0 first stage, 1 middle stages, 2 final stage (+ inspection)
     const int *status;
     Partial *partials; // border tiles first, then interior tiles
     const int *tiles;  // border tile ids (64x4 tiles, gntx per row)
@@ -348,12 +348,19 @@ __device__ void block_reduce(Partial &p, Partial *sh) {
 }
 
 // energy (PointKernels2D.hpp:181-184) and inspection terms (CurvilinearSolver2D.cpp:102-111) of one node
-template <class R>
+// FAST: one FP64 division (xz^2/(2mu) = xz^2 * 4(lambda+mu) / (8 mu (lambda+mu))), same value to rounding
+template <class R, bool FAST = false>
 __device__ inline void node_stats(R rho, R lam, R mu, R jac, R h, const R *u, bool surface, const double *nrm, Partial &p) {
     const double l = lam, mm = mu, xx = u[2], zz = u[3], xz = u[4], vx = u[0], vz = u[1];
-    const double e = (.5 * (double)rho * (vx * vx + vz * vz) +
-                      ((l + 2 * mm) * (xx * xx + zz * zz) - 2 * l * xx * zz) / (8 * mm * (l + mm)) + xz * xz / (2 * mm)) *
-                     (double)jac * (double)h * (double)h;
+    double e;
+    if constexpr (FAST)
+        e = (.5 * (double)rho * (vx * vx + vz * vz) +
+             ((l + 2 * mm) * (xx * xx + zz * zz) - 2 * l * xx * zz + xz * xz * (4 * (l + mm))) / (8 * mm * (l + mm))) *
+            (double)jac * (double)h * (double)h;
+    else
+        e = (.5 * (double)rho * (vx * vx + vz * vz) +
+             ((l + 2 * mm) * (xx * xx + zz * zz) - 2 * l * xx * zz) / (8 * mm * (l + mm)) + xz * xz / (2 * mm)) *
+            (double)jac * (double)h * (double)h;
     const R v = sqrt(u[0] * u[0] + u[1] * u[1]);
     p.energy += e;
     p.velocity = fmax(p.velocity, (double)v);
@@ -442,6 +449,15 @@ template <class R> __device__ inline void vstore(R *p, typename Lane<R>::V v, in
     else *d = v;
 }
 
+// Interior arithmetic.  FP64 keeps the reference's operation order exactly (separate multiply and add,
+// true divisions), so FP64 runs reproduce the reference to the bit.  FP32 is held to a tolerance, not
+// to the bit, so its interior fuses multiply-adds and replaces repeated divisions by one reciprocal.
+template <class R> constexpr bool kFast = sizeof(R) == 4;
+template <class R> __device__ inline R mad(R a, R b, R c) {
+    if constexpr (kFast<R>) return __builtin_fmaf(a, b, c);
+    else return a * b + c;
+}
+
 template <class R, int DX, int DZ, int MECH>
 __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Partial *sh) {
     using V = typename Lane<R>::V;
@@ -475,7 +491,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
             for (int j = 0; j < NV; ++j) {
                 R r = 0;
 #pragma unroll
-                for (int l = 0; l < 5; ++l) r += coef<R, DX>(l) * row[HV * NV + j + offs<DX>(l)];
+                for (int l = 0; l < 5; ++l) r = mad(coef<R, DX>(l), row[HV * NV + j + offs<DX>(l)], r);
                 xi[f][j] = over_h(r, h, m.hinv);
             }
             // z: five rows of the same columns
@@ -486,19 +502,26 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
             for (int j = 0; j < NV; ++j) {
                 R r = 0;
 #pragma unroll
-                for (int l = 0; l < 5; ++l) r += coef<R, DZ>(l) * col[l][j];
+                for (int l = 0; l < 5; ++l) r = mad(coef<R, DZ>(l), col[l][j], r);
                 eta[f][j] = over_h(r, h, m.hinv);
             }
         }
-        // separable metrics in FP64 (bit-identical to the host arrays) or the stored arrays
+        // separable metrics in FP64 (bit-identical to the host arrays; the FP32 path takes one FP64
+        // reciprocal, within 1 FP32 ulp of the stored values) or the stored arrays
         R a[NV], b[NV], jac[NV];
         if (m.metric) {
             const double fk = m.fade[k], fdk = m.fadeder[k];
 #pragma unroll
             for (int j = 0; j < NV; ++j) {
                 const double zx = m.tslope[i0 + j] * fk, jj = 1 + m.tz_[i0 + j] * fdk;
-                a[j] = (R)(-zx / jj);
-                b[j] = (R)(1 / jj);
+                if constexpr (kFast<R>) {
+                    const double inv = 1 / jj;
+                    a[j] = (R)(-zx * inv);
+                    b[j] = (R)inv;
+                } else {
+                    a[j] = (R)(-zx / jj);
+                    b[j] = (R)(1 / jj);
+                }
                 jac[j] = (R)jj;
             }
         } else {
@@ -515,33 +538,47 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
             R dx[5], dz[5];
 #pragma unroll
             for (int f = 0; f < 5; ++f) {
-                dx[f] = xi[f][j] + a[j] * eta[f][j];
+                dx[f] = mad(a[j], eta[f][j], xi[f][j]);
                 dz[f] = b[j] * eta[f][j];
             }
             R o0, o1;
-            if (m.cpml) { // outside the strips the ADE stretch is the identity, but the momentum keeps its split form
-                o0 = xi[2][j] / rho + (a[j] * eta[2][j] + b[j] * eta[4][j]) / rho;
-                o1 = xi[4][j] / rho + (a[j] * eta[4][j] + b[j] * eta[3][j]) / rho;
+            if constexpr (kFast<R>) {
+                // 1/rho once; the split (CFS-PML) and plain momentum forms agree up to rounding here
+                const R rinv = (R)1 / rho;
+                o0 = (xi[2][j] + mad(a[j], eta[2][j], b[j] * eta[4][j])) * rinv;
+                o1 = (xi[4][j] + mad(a[j], eta[4][j], b[j] * eta[3][j])) * rinv;
+                if (src) {
+                    const ptrdiff_t qq = q + j;
+                    const R g0 = m.has_grad ? m.grad0[qq] : m.src[qq] * m.dir0;
+                    const R g1 = m.has_grad ? m.grad1[qq] : m.src[qq] * m.dir2;
+                    o0 = mad(A.forcing * g0, rinv, o0);
+                    o1 = mad(A.forcing * g1, rinv, o1);
+                }
             } else {
-                o0 = (dx[2] + dz[4]) / rho;
-                o1 = (dx[4] + dz[3]) / rho;
-            }
-            if (src) {
-                const ptrdiff_t qq = q + j;
-                if (m.has_grad) {
-                    o0 += A.forcing * m.grad0[qq] / rho;
-                    o1 += A.forcing * m.grad1[qq] / rho;
+                if (m.cpml) { // outside the strips the ADE stretch is the identity, but the momentum keeps its split form
+                    o0 = xi[2][j] / rho + (a[j] * eta[2][j] + b[j] * eta[4][j]) / rho;
+                    o1 = xi[4][j] / rho + (a[j] * eta[4][j] + b[j] * eta[3][j]) / rho;
                 } else {
-                    const R sw = m.src[qq];
-                    o0 += A.forcing * (sw * m.dir0) / rho;
-                    o1 += A.forcing * (sw * m.dir2) / rho;
+                    o0 = (dx[2] + dz[4]) / rho;
+                    o1 = (dx[4] + dz[3]) / rho;
+                }
+                if (src) {
+                    const ptrdiff_t qq = q + j;
+                    if (m.has_grad) {
+                        o0 += A.forcing * m.grad0[qq] / rho;
+                        o1 += A.forcing * m.grad1[qq] / rho;
+                    } else {
+                        const R sw = m.src[qq];
+                        o0 += A.forcing * (sw * m.dir0) / rho;
+                        o1 += A.forcing * (sw * m.dir2) / rho;
+                    }
                 }
             }
             const R lambda = vlam[j], mu = vmu[j], div = dx[0] + dz[1];
             kv[0][j] = o0;
             kv[1][j] = o1;
-            kv[2][j] = lambda * div + 2 * mu * dx[0];
-            kv[3][j] = lambda * div + 2 * mu * dz[1];
+            kv[2][j] = mad(lambda, div, 2 * mu * dx[0]);
+            kv[3][j] = mad(lambda, div, 2 * mu * dz[1]);
             kv[4][j] = mu * (dz[0] + dx[1]);
             if constexpr (MECH > 0) kv[5][j] = lambda * div;
         }
@@ -572,16 +609,33 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
 #pragma unroll
         for (int f = 0; f < F; ++f) {
             const size_t o = (size_t)f * cnt + q;
-            const V vb = A.kind == 0 && f < 5 ? centre[f < 5 ? f : 0] : vload(A.base + o); // stage 1: base is the input
-            V va;
-            if (A.kind != 0) va = vload(A.acc + o);
             V vo, vacc;
+            if constexpr (kFast<R>) {
+                // FP32: the accumulator holds Y = U^n + dt/6 k1 + dt/3 k2 + dt/3 k3, so the last stage
+                // reads Y and the stage input only (the compulsory 3S traffic), not U^n as well
+                const V vb = A.kind == 2 ? V{} : A.kind == 0 && f < 5 ? centre[f < 5 ? f : 0] : vload(A.base + o);
+                const V vy = A.kind == 0 ? vb : vload(A.acc + o);
 #pragma unroll
-            for (int j = 0; j < NV; ++j) {
-                R acc = A.kind == 0 ? (R)0 : va[j];
-                nxt[f][j] = combine(A.kind, vb[j], kv[f][j], A.w, acc);
-                vo[j] = nxt[f][j];
-                vacc[j] = acc;
+                for (int j = 0; j < NV; ++j) {
+                    if (A.kind == 2) {
+                        nxt[f][j] = mad(A.wy, kv[f][j], vy[j]);
+                    } else {
+                        nxt[f][j] = mad(A.w, kv[f][j], vb[j]);
+                        vacc[j] = mad(A.wy, kv[f][j], vy[j]);
+                    }
+                    vo[j] = nxt[f][j];
+                }
+            } else {
+                const V vb = A.kind == 0 && f < 5 ? centre[f < 5 ? f : 0] : vload(A.base + o); // stage 1: base is the input
+                V va;
+                if (A.kind != 0) va = vload(A.acc + o);
+#pragma unroll
+                for (int j = 0; j < NV; ++j) {
+                    R acc = A.kind == 0 ? (R)0 : va[j];
+                    nxt[f][j] = combine(A.kind, vb[j], kv[f][j], A.w, acc);
+                    vo[j] = nxt[f][j];
+                    vacc[j] = acc;
+                }
             }
             vstore(A.out + o, vo, A.nt);
             if (A.kind != 2) vstore(A.acc + o, vacc, A.nt);
@@ -595,7 +649,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
                     if (!isfinite(nxt[f][j])) p.bad = 1;
                     if (f < 5) u[f] = nxt[f][j];
                 }
-                node_stats(vrho[j], vlam[j], vmu[j], jac[j], h, u, false, nullptr, p);
+                node_stats<R, kFast<R>>(vrho[j], vlam[j], vmu[j], jac[j], h, u, false, nullptr, p);
             }
         }
     }
@@ -608,7 +662,11 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
 // One launch per RK stage: blocks [0, gblocks) take the border tiles (gblocks is ngeneral rounded up to
 // a multiple of the 8 XCDs, so the interior blocks keep their XCD-contiguous band mapping), the rest
 // take the interior tiles.  A single in-order launch avoids cross-stream joins between stages.
-template <class R, int DX, int DZ, int MECH> __global__ __launch_bounds__(THREADS) void stage_kernel(StageArgs<R> A) {
+// The FP32 elastic stage fits 96 VGPRs without spilling, which gives 5 waves per SIMD instead of 4.
+template <class R, int MECH> constexpr int kWavesPerEU = sizeof(R) == 4 && MECH == 0 ? 5 : 1;
+template <class R, int DX, int DZ, int MECH>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<R, MECH>))) void stage_kernel(
+    StageArgs<R> A) {
     __shared__ Partial sh[THREADS];
     if (*A.status) return;
     const int b = blockIdx.x;

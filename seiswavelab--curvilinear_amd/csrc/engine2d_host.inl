@@ -21,7 +21,7 @@ template <class R> struct Engine2D {
     int rtx = 0, nInterior = 0;   // interior tiles (64*NV x 4 nodes)
     int nPartials = 0, nLevel2 = 0;
     hipEvent_t fork[4]{}, join[4]{};
-    int cur = 0, prev = 0, diag_rows = 0, registers = 0, nt = 1;
+    int cur = 0, prev = 0, diag_rows = 0, registers = 0, nt = 1, metricOk = 0;
     size_t local_bytes = 0;
     std::vector<Partial> host_partials;
     // optional per-launch timing with HIP events (interior stage kernel and step finalisation)
@@ -72,7 +72,7 @@ template <class R> struct Engine2D {
         for (size_t j = 0; j + 1 < eventsUsed; j += 2) {
             float ms = 0;
             check(hipEventElapsedTime(&ms, events[j], events[j + 1]), "event elapsed");
-            if (eventKind[j / 2] == 0) stageMs += ms, ++stageLaunches;
+            if (eventKind[j / 2] == 0) stageMs += ms, stageLaunches += 4; // one pair spans a step's 4 stages
             else finalMs += ms, ++finalLaunches;
         }
         eventsUsed = 0;
@@ -102,7 +102,8 @@ template <class R> struct Engine2D {
         {
             int e = 0;
             const double frac = std::frexp(d.h, &e); // h = frac * 2^e, power of two iff frac == 0.5
-            m.hinv = frac == 0.5 ? (R)(1 / d.h) : (R)0;
+            // FP32 (held to a tolerance) always multiplies by 1/h; FP64 only where that is exact
+            m.hinv = frac == 0.5 || kFast<R> ? (R)(1 / d.h) : (R)0;
         }
         for (int l = 0; l < 3; ++l) m.rates[l] = (R)d.rates[l];
         m.dir0 = (R)d.source_direction[0], m.dir2 = (R)d.source_direction[2];
@@ -183,7 +184,7 @@ template <class R> struct Engine2D {
         m.tslope = put_plain<double>(25, d.surface_slope, nx);
         m.fade = put_plain<double>(26, d.fade, nz);
         m.fadeder = put_plain<double>(27, d.fade_derivative, nz);
-        m.metric = 1;
+        m.metric = metricOk = 1;
     }
 
     // Interior rectangle: depth >= 4, outside every CFS-PML slot, zero sponge damping; x-extent
@@ -254,11 +255,13 @@ template <class R> struct Engine2D {
         level2.allocate((size_t)nLevel2 * sizeof(Partial));
     }
 
-    template <int DX, int DZ> void launch_both(StageArgs<R> a, int) {
+    // slot = RK stage 0..3; with profiling on, one event pair brackets the four back-to-back stage launches
+    // (an event between launches would itself add a dispatch gap)
+    template <int DX, int DZ> void launch_both(StageArgs<R> a, int slot) {
         a.tiles = generalTiles.as<int>(), a.ngeneral = nGeneral, a.gblocks = (nGeneral + 7) / 8 * 8, a.gntx = ntx;
         a.ninterior = nInterior, a.rtx = rtx, a.nt = nt;
         const int blocks = a.gblocks + nInterior;
-        mark(0, true);
+        if (slot == 0) mark(0, true);
         switch (mech) {
         case 0: stage_kernel<R, DX, DZ, 0><<<blocks, THREADS, 0, stream>>>(a); break;
         case 1: stage_kernel<R, DX, DZ, 1><<<blocks, THREADS, 0, stream>>>(a); break;
@@ -266,7 +269,7 @@ template <class R> struct Engine2D {
         default: stage_kernel<R, DX, DZ, 3><<<blocks, THREADS, 0, stream>>>(a); break;
         }
         check(hipGetLastError(), "2D stage launch");
-        mark(0, false);
+        if (slot == 3) mark(0, false);
     }
     void launch_stage(int dx, int dz, const StageArgs<R> &a, int slot) {
         if (dx == 0 && dz == 0) launch_both<0, 0>(a, slot);
@@ -299,6 +302,7 @@ template <class R> struct Engine2D {
             a.count = count;
             a.forcing = (R)(scale * pulse(times[s], frequency));
             a.w = weights[s];
+            a.wy = (R)(s == 0 || s == 3 ? dt / 6 : dt / 3);
             a.kind = s == 0 ? 0 : s == 3 ? 2 : 1;
             a.status = status.as<int>();
             a.partials = partials.as<Partial>();
@@ -432,7 +436,8 @@ template <class R> struct Engine2D {
         for (int j = 0; j < count_steps; ++j) {
             const int s = first + j;
             const int n = enqueue_step(c, dt, frequency, scale, s);
-            mark(1, true);
+            const bool timed = j % 32 == 0; // finalisation is timed on every 32nd step only (events add gaps)
+            if (timed) mark(1, true);
             reduce_kernel<<<nLevel2, THREADS, 0, stream>>>(partials.as<Partial>(), nPartials, level2.as<Partial>(),
                                                           status.as<int>());
             FinalArgs<R> f{};
@@ -442,7 +447,7 @@ template <class R> struct Engine2D {
             f.state = S(n), f.probes = probes.as<long long>(), f.nrec = nrec, f.row = j, f.step = s;
             f.dt = dt, f.frequency = frequency;
             finalize_kernel<R><<<1, THREADS, 0, stream>>>(f);
-            mark(1, false);
+            if (timed) mark(1, false);
             check(hipGetLastError(), "2D finalize launch");
             c = n;
             states[j + 1] = c;
@@ -553,6 +558,7 @@ int swl_engine2d_set_option(swl_engine2d *e, const char *name, int value) {
         const std::string key = name ? name : "";
         e->with([&](auto &x) {
             if (key == "nontemporal_stores") x.nt = value != 0;
+            else if (key == "stored_metrics") x.m.metric = value == 0 && x.metricOk; // read a, b, J from memory
             else throw std::runtime_error("unknown engine option: " + key);
         });
     });
