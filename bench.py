@@ -74,11 +74,11 @@ def roofline(res, prec):
     avg_ms = t["stage_ms"] / max(t["stage_launches"], 1)
     # the dominant kernel is the fused RK-stage kernel; one launch updates every computational node
     # (interior and border tiles), and a step is four such launches
-    per_launch = ALGO_BYTES[prec] / 4 * t["interior_nodes"]
+    per_launch = ALGO_BYTES[prec] / 4 * t["nodes_per_launch"]
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_ms": round(avg_ms, 5),
-            "algorithmic_bytes_per_launch": per_launch, "nodes_per_launch": t["interior_nodes"],
+            "algorithmic_bytes_per_launch": per_launch, "nodes_per_launch": t["nodes_per_launch"],
             "metric_on_the_fly": t["metric_on_the_fly"], "vgprs": t["vgprs"]}
 
 
@@ -109,7 +109,7 @@ def cpu_baseline(pkg, params, steps=10, warmup=2):
 
 def traffic_from_profile(prec):
     """HBM bytes per stage launch from the committed rocprofv3 PMC summary, if present."""
-    path = ROOT / "profiles" / f"stage2d_config2_{prec}_pmc.json"
+    path = ROOT / "profiles" / f"stage_kernel_elastic2d_5061x2031_{prec}_pmc.json"
     if not path.exists():
         return None
     try:
@@ -134,16 +134,16 @@ def main():
                     help="size of the plain copy used as the sustained-bandwidth reference (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; shots and domains shard per GPU by design "
-              "(see DESIGN.md), and multi-GPU runs are not part of it.", file=sys.stderr)
+        print("bench.py measures one GPU only: a shot (or a batch of shots) runs on one device, and "
+              "runs across several GPUs are not part of this benchmark.", file=sys.stderr)
         sys.exit(2)
     pkg = _package()
     from seiswavelab_curvilinear_amd import workloads
 
     t = time.perf_counter()
-    w = workloads.config2(seed=2, precision=pkg.Precision.Float32, nx=args.nx, nz=args.nz,
+    w = workloads.grf_terrain_5001x2001(seed=2, precision=pkg.Precision.Float32, nx=args.nx, nz=args.nz,
                           steps=args.warmup + args.steps + 10)
-    log(f"[bench] config-2 model generated in {time.perf_counter() - t:.1f} s")
+    log(f"[bench] 5001x2001 GRF-terrain model generated in {time.perf_counter() - t:.1f} s")
     copy = None
     if args.probe_bytes > 0:
         ms = pkg.native.stream_probe(args.probe_bytes, 10)
@@ -161,6 +161,7 @@ def main():
         w.params.precision = pkg.Precision.Float64
         r64 = measure(pkg, w.params, args.fp64_steps, min(args.warmup, 10), "FP64", options)
         roof64 = roofline(r64, "f64")
+        roof64["traffic"] = traffic_from_profile("f64")
         extra["fp64"] = {"value": round(r64["nodes"] * args.fp64_steps / r64["wall"] / 1e9, 4),
                          "ms_per_step": round(1e3 * r64["wall"] / args.fp64_steps, 4), "roofline": roof64}
         w.params.precision = pkg.Precision.Float32

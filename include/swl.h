@@ -181,15 +181,16 @@ typedef struct swl_run2d {
 } swl_run2d;
 int swl_engine2d_run(swl_engine2d *e, int first_step, int count, double dt, double frequency, double scale,
                      double peak_energy, double *diag, double *samples, swl_run2d *result);
-/* Per-launch timing with HIP events on the engine's stream during swl_engine2d_run (no
-   reference counterpart; the reference times whole steps on the host).  enable resets the
-   counters.  timing: stage-kernel ms, stage launches, finalize ms, finalize launches,
-   tiles per launch, stage-kernel VGPRs. */
+/* Timing with HIP events on the engine's stream during swl_engine2d_run (no reference
+   counterpart; the reference times whole steps on the host).  One event pair brackets the four
+   back-to-back stage launches of a step; the finalisation is timed on every 32nd step.
+   enable resets the counters. */
 int swl_engine2d_profile(swl_engine2d *e, int enable);
-/* timing: interior stage-kernel ms, launches, finalisation ms, launches, interior tiles, interior
-   VGPRs, interior nodes per launch, border tiles, on-the-fly metrics (0/1). */
+/* out[9]: stage-kernel ms, stage launches, finalisation ms, finalisation launches, interior tiles,
+   stage-kernel VGPRs, computational nodes per stage launch, border tiles, on-the-fly metrics (0/1). */
 int swl_engine2d_kernel_timing(swl_engine2d *e, double *out);
-/* Engine tuning switches: "nontemporal_stores" (default 1) for the interior kernel's outputs. */
+/* Engine tuning switches: "nontemporal_stores" (default 1) for the interior outputs;
+   "stored_metrics" (default 0) reads a, b, J from memory instead of recomputing them. */
 int swl_engine2d_set_option(swl_engine2d *e, const char *name, int value);
 /* Measurement utility (no reference counterpart): `repeats` plain streaming copies of `bytes`
    (read `bytes`, write `bytes`) on `device`; ms_per_copy = mean HIP-event time per copy.  Used to

@@ -182,7 +182,7 @@ class Simulation:
         o = np.zeros(9)
         native.call("swl_sim2d_kernel_timing", self._h, _dp(o))
         return {"stage_ms": o[0], "stage_launches": int(o[1]), "finalize_ms": o[2], "finalize_launches": int(o[3]),
-                "interior_tiles": int(o[4]), "vgprs": int(o[5]), "interior_nodes": int(o[6]),
+                "interior_tiles": int(o[4]), "vgprs": int(o[5]), "nodes_per_launch": int(o[6]),
                 "border_tiles": int(o[7]), "metric_on_the_fly": bool(o[8])}
 
     def set_option(self, name: str, value: int):

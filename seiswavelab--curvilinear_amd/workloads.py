@@ -15,7 +15,7 @@ import numpy as np
 
 from .params import Absorber, GslsSettings, Layer, Parameters, Precision, SourceKind, SourceMechanism
 
-CONFIG1_LAYERS = [Layer(60, 800, 400, 1800), Layer(140, 1800, 900, 2100), Layer(0, 3000, 1700, 2400)]
+THREE_LAYERS = [Layer(60, 800, 400, 1800), Layer(140, 1800, 900, 2100), Layer(0, 3000, 1700, 2400)]
 
 
 def grf_1d(n: int, h: float, corr: float, rms: float, rng: np.random.Generator) -> np.ndarray:
@@ -55,7 +55,7 @@ def layered_base(nz: int, h: float, layers):
     return vp, vs, rho
 
 
-def config1(seed: int = 1, precision: Precision = Precision.Float64) -> Parameters:
+def sine_terrain_500x200(seed: int = 1, precision: Precision = Precision.Float64) -> Parameters:
     """500x200 at h=2: z_s(x)=10 sin(2 pi x/400)+4 sin(2 pi x/137+phase), 3 layers, 30 Hz vertical
     force at the surface midpoint, 20-node CFS-PML, CFL 0.5, 2000 steps, Vz/Vx at every surface node."""
     rng = np.random.default_rng(seed)
@@ -66,7 +66,7 @@ def config1(seed: int = 1, precision: Precision = Precision.Float64) -> Paramete
     return Parameters(precision=precision, nx=nx, nz=nz, spacing=h, frequency=30, duration=1.0, cfl=.5,
                       maxSteps=2000, sourceX=-1, sourceZ=0, source=SourceMechanism(SourceKind.DirectionalForce),
                       absorber=Absorber.CfsPml, spongeX=20, spongeZ=20, receiverStart=0, receiverCount=nx,
-                      receiverStride=1, receiverDepth=0, layered=True, layers=list(CONFIG1_LAYERS),
+                      receiverStride=1, receiverDepth=0, layered=True, layers=list(THREE_LAYERS),
                       importedSurface=True, surfaceSamples=np.stack([x, z], 1))
 
 
@@ -77,7 +77,7 @@ class Heterogeneous2D:
     note: str
 
 
-def config2(seed: int = 2, precision: Precision = Precision.Float32, nx: int = 5001, nz: int = 2001,
+def grf_terrain_5001x2001(seed: int = 2, precision: Precision = Precision.Float32, nx: int = 5001, nz: int = 2001,
             steps: int = 4000, source_z: int = 5) -> Heterogeneous2D:
     """5001x2001 at h=2: GRF topography (60 m correlation, 25 m RMS), three layers each perturbed by
     independent 5 % RMS GRFs (20 m correlation) on Vp and Vs with Vp/Vs >= 1.6, 30 Hz Ricker
@@ -87,7 +87,7 @@ def config2(seed: int = 2, precision: Precision = Precision.Float32, nx: int = 5
     h = 2.0
     topo = grf_1d(nx, h, 60.0, 25.0, rng)
     x = np.arange(nx) * h
-    bvp, bvs, brho = layered_base(nz, h, CONFIG1_LAYERS)
+    bvp, bvs, brho = layered_base(nz, h, THREE_LAYERS)
     dp = grf_2d(nz, nx, h, 20.0, 0.05, rng)
     ds = grf_2d(nz, nx, h, 20.0, 0.05, rng)
     vp = bvp[:, None] * (1 + dp)
@@ -99,9 +99,9 @@ def config2(seed: int = 2, precision: Precision = Precision.Float32, nx: int = 5
                    sourceX=-1, sourceZ=source_z, source=SourceMechanism(SourceKind.IsotropicMoment),
                    momentPerLength=2e6, absorber=Absorber.CfsPml, spongeX=30, spongeZ=30, externalAbsorption=True,
                    receiverStart=0, receiverCount=min(1001, (nx - 1) // 5 + 1), receiverStride=5, receiverDepth=0,
-                   layered=True, layers=list(CONFIG1_LAYERS), importedSurface=True,
+                   layered=True, layers=list(THREE_LAYERS), importedSurface=True,
                    surfaceSamples=np.stack([x, topo], 1), nodeVp=vp, nodeVs=vs, nodeDensity=rho)
-    return Heterogeneous2D(p, seed, "config-2 model: seeded GRF topography and velocities")
+    return Heterogeneous2D(p, seed, "5001x2001 GRF terrain: seeded GRF topography and velocities")
 
 
 def computational_nodes(p: Parameters) -> int:

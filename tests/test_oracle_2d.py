@@ -176,7 +176,7 @@ def test_moment_pattern_conservative():
 
 
 REF_CASES = {
-    "config1_small": dict(nx=120, nz=60, spacing=2, frequency=30, duration=1, cfl=.5, maxSteps=120, sourceX=-1,
+    "sine_terrain_small": dict(nx=120, nz=60, spacing=2, frequency=30, duration=1, cfl=.5, maxSteps=120, sourceX=-1,
                           sourceZ=0, absorber=Absorber.CfsPml, spongeX=20, spongeZ=20, receiverStart=0,
                           receiverCount=120, receiverStride=1, receiverDepth=0, layered=True),
     "moment_external_pml": dict(nx=90, nz=50, spacing=2, relief=4, wavelength=100, frequency=30, maxSteps=150,
@@ -196,7 +196,7 @@ REF_CASES = {
 def ref_case(name):
     kw = dict(REF_CASES[name])
     p = Parameters(**kw)
-    if name == "config1_small":
+    if name == "sine_terrain_small":
         phase = 1.234
         x = np.arange(0, 240, 2.0)
         p.importedSurface = True
