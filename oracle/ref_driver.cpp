@@ -6,6 +6,7 @@
 // copies results out through the same function shapes as oracle.h (prefix ref2d_/ref3d_).
 #include "oracle.h"
 #include "Seis2D/CurvilinearSolver2D.hpp"
+#include "Seis3D/AbsorbingBoundary3D.hpp"
 #include "Seis3D/CurvilinearSolver.hpp"
 #include "two_d/CfsPml2D.hpp"
 #include "two_d/PointKernels2D.hpp"
@@ -154,5 +155,123 @@ int64_t ref2d_pml_slot(int nx_pad, int nz_pad, int width_x, int width_z, int64_t
     m.nx = nx_pad, m.nz = nz_pad, m.cpmlWidthX = width_x, m.cpmlWidthZ = width_z;
     m.cpmlCount = 4 * (2 * static_cast<size_t>(width_x) * (nz_pad - 6) + static_cast<size_t>(width_z) * (nx_pad - 6));
     return S2::Point::pmlSlot(m, static_cast<size_t>(q), axis);
+}
+
+// ---- 3D: CpuSimulation3D on the CPU backend ----
+void *ref3d_create(const orc_params3d *o, char *error, int len) {
+    S3::SimulationParameters p;
+    p.nx = o->nx, p.ny = o->ny, p.nz = o->nz, p.spacing = o->spacing, p.vp = o->vp, p.vs = o->vs, p.density = o->density;
+    p.layeredMaterial = o->layered != 0;
+    p.layerDepthMode = static_cast<S3::LayerDepthMode3D>(o->layer_depth_mode);
+    p.layers.clear();
+    for (int l = 0; l < o->n_layers; ++l)
+        p.layers.push_back({"layer", o->layer_thickness[l], o->layer_vp[l], o->layer_vs[l], o->layer_density[l]});
+    p.dominantFrequency = o->dominant_frequency;
+    p.sourceX = o->source_x, p.sourceY = o->source_y, p.sourceZ = o->source_z;
+    p.source.kind = o->source_kind ? SeisWave::SourceKind::IsotropicMoment : SeisWave::SourceKind::DirectionalForce;
+    p.source.direction = {o->source_direction[0], o->source_direction[1], o->source_direction[2]};
+    p.sourceForce = o->source_force, p.sourceMoment = o->source_moment;
+    p.duration = o->duration, p.maxSteps = o->max_steps, p.cflSafety = o->cfl;
+    p.topographyAmplitude = o->topography_amplitude, p.topographyWavelength = o->topography_wavelength;
+    p.terrainTransitionDepth = o->terrain_transition_depth;
+    if (o->dem_nx > 0 && o->dem_elevations) {
+        auto &g = p.surfaceGrid;
+        g.nx = o->dem_nx, g.ny = o->dem_ny, g.spacing = o->dem_spacing, g.originX = o->dem_origin_x;
+        g.originY = o->dem_origin_y, g.datum = o->dem_datum;
+        g.elevations.assign(o->dem_elevations, o->dem_elevations + static_cast<size_t>(o->dem_nx) * o->dem_ny);
+    }
+    auto &r = p.receiverArray;
+    r.kind = static_cast<S3::ReceiverArrayKind3D>(o->receiver_kind);
+    r.startX = o->receiver_start_x, r.startY = o->receiver_start_y, r.z = o->receiver_z;
+    r.countX = o->receiver_count_x, r.countY = o->receiver_count_y, r.strideX = o->receiver_stride_x;
+    r.strideY = o->receiver_stride_y;
+    p.absorbingBoundary.enabled = o->absorbing != 0;
+    p.absorbingBoundary.cells = {o->absorbing_cells[0], o->absorbing_cells[1], o->absorbing_cells[2]};
+    p.absorbingBoundary.strength = o->absorbing_strength;
+    auto s = std::make_unique<S3::CpuSimulation3D>(S3::BackendKind3D::Cpu);
+    std::string e;
+    if (!s->initialize(p, e, S3::ComputeSettings3D{o->threads, -1})) {
+        copyError(e, error, len);
+        return nullptr;
+    }
+    copyError("", error, len);
+    return s.release();
+}
+void ref3d_destroy(void *h) { delete static_cast<S3::CpuSimulation3D *>(h); }
+int ref3d_advance(void *h, int count, char *error, int len) {
+    std::string e;
+    const bool ok = static_cast<S3::CpuSimulation3D *>(h)->advance(count, e);
+    copyError(e, error, len);
+    return ok ? 1 : 0;
+}
+int ref3d_step(void *h) { return static_cast<S3::CpuSimulation3D *>(h)->currentStep(); }
+int ref3d_total(void *h) { return static_cast<S3::CpuSimulation3D *>(h)->totalSteps(); }
+double ref3d_dt(void *h) { return static_cast<S3::CpuSimulation3D *>(h)->dt(); }
+void ref3d_layout(void *h, int64_t *o) {
+    auto &s = *static_cast<S3::CpuSimulation3D *>(h);
+    const auto &g = s.grid();
+    o[0] = g.nx, o[1] = g.ny, o[2] = g.nz, o[3] = static_cast<int64_t>(s.receivers().size());
+    o[4] = s.receivers().empty() ? 0 : static_cast<int64_t>(s.receivers()[0].samples.size());
+}
+// same order as orc3d_diagnostics; the source-window peak is private to the reference (0 here)
+void ref3d_diagnostics(void *h, double *o) {
+    const auto &d = static_cast<S3::CpuSimulation3D *>(h)->diagnostics();
+    const double v[18] = {d.elasticEnergy,  d.peakEnergy,     0.0,            d.maxAbsVelocity, d.maxTractionRateRelative,
+                          d.allFinite ? 1.0 : 0.0, d.minJacobian, d.maxMetricNorm, d.minCellLength, d.maxCellEdge,
+                          d.waveDtLimit,    d.dampingDtLimit, d.sourceDtLimit, d.displayPeaks[0], d.displayPeaks[1],
+                          d.displayPeaks[2], d.displayPeaks[3], d.displayPeaks[4]};
+    std::copy(v, v + 18, o);
+}
+void ref3d_receivers(void *h, double *vx, double *vy, double *vz, double *position, int64_t *index) {
+    const auto &rs = static_cast<S3::CpuSimulation3D *>(h)->receivers();
+    for (size_t r = 0; r < rs.size(); ++r) {
+        const size_t n = rs[r].samples.size();
+        if (vx) std::copy(rs[r].samplesVx.begin(), rs[r].samplesVx.end(), vx + r * n);
+        if (vy) std::copy(rs[r].samplesVy.begin(), rs[r].samplesVy.end(), vy + r * n);
+        if (vz) std::copy(rs[r].samples.begin(), rs[r].samples.end(), vz + r * n);
+        if (position) std::copy(rs[r].position.begin(), rs[r].position.end(), position + 3 * r);
+        if (index) index[r] = static_cast<int64_t>(rs[r].gridIndex);
+    }
+}
+void ref3d_field(void *h, int f, double *out) {
+    const auto &v = static_cast<S3::CpuSimulation3D *>(h)->field(static_cast<S3::Field3D>(f));
+    std::copy(v.begin(), v.end(), out);
+}
+// 0 metrics (9N), 1 jacobian, 2 normals (3N), 3 coordinates (3N), 7 sponge rate of every physical
+// node (reference absorbingRate3D), 12..14 layered Vp, Vs, density of the export snapshot.
+int ref3d_model(void *h, int which, double *out) {
+    auto &s = *static_cast<S3::CpuSimulation3D *>(h);
+    const auto &g = s.grid();
+    const size_t n = g.coordinates.size();
+    switch (which) {
+    case 0: for (size_t q = 0; q < n; ++q) std::copy(g.inverseMetrics[q].begin(), g.inverseMetrics[q].end(), out + 9 * q); return 1;
+    case 1: std::copy(g.jacobian.begin(), g.jacobian.end(), out); return 1;
+    case 2: for (size_t q = 0; q < n; ++q) std::copy(g.surfaceNormal[q].begin(), g.surfaceNormal[q].end(), out + 3 * q); return 1;
+    case 3: for (size_t q = 0; q < n; ++q) std::copy(g.coordinates[q].begin(), g.coordinates[q].end(), out + 3 * q); return 1;
+    case 7: {
+        const auto &p = s.parameters();
+        double maxVp = p.vp;
+        if (p.layeredMaterial) {
+            maxVp = 0;
+            for (const auto &l : p.layers) maxVp = std::max(maxVp, l.vp);
+        }
+        std::fill(out, out + n, 0.0);
+        for (int k = g.ghost; k < g.nz - g.ghost; ++k)
+            for (int j = g.ghost; j < g.ny - g.ghost; ++j)
+                for (int i = g.ghost; i < g.nx - g.ghost; ++i)
+                    out[g.index(i, j, k)] = S3::absorbingRate3D(p, i - g.ghost, j - g.ghost, k - g.ghost, maxVp);
+        return 1;
+    }
+    case 12:
+    case 13:
+    case 14: {
+        const auto snap = s.snapshot();
+        const auto &v = which == 12 ? snap.materialVp : which == 13 ? snap.materialVs : snap.materialDensity;
+        if (v.size() != n) return 0;
+        std::copy(v.begin(), v.end(), out);
+        return 1;
+    }
+    default: return 0;
+    }
 }
 } // extern "C"

@@ -161,3 +161,91 @@ def rel_l2(ref, val):
     ref = np.asarray(ref, dtype=np.float64).ravel()
     val = np.asarray(val, dtype=np.float64).ravel()
     return float(np.sqrt(np.sum((ref - val) ** 2) / max(np.sum(ref * ref), 1e-300)))
+
+
+DIAG3D = ["energy", "peakEnergy", "sourcePeakEnergy", "maxAbsVelocity", "maxTractionRateRelative", "allFinite",
+          "minJacobian", "maxMetricNorm", "minCellLength", "maxCellEdge", "waveDtLimit", "dampingDtLimit",
+          "sourceDtLimit", "peakVz", "peakVx", "peakVy", "peakV", "peakPressure"]
+
+
+class CpuSession3D:
+    """Drives orc3d_* / ref3d_* (CpuSimulation3D on the CPU backend)."""
+
+    def __init__(self, kind: str, params):
+        from seiswavelab_curvilinear_amd.params import to_c3d
+        self.lib = load(kind)
+        if self.lib is None or not hasattr(self.lib, f"{kind}3d_create"):
+            raise FileNotFoundError(kind)
+        self.pre = f"{kind}3d_"
+        c, self._keep = to_c3d(params)
+        self._c = c
+        err = C.create_string_buffer(512)
+        self.h = getattr(self.lib, self.pre + "create")(C.byref(c), err, 512)
+        self.error = err.value.decode()
+        if not self.h:
+            raise ValueError(self.error)
+
+    def _f(self, name):
+        return getattr(self.lib, self.pre + name)
+
+    def close(self):
+        if self.h:
+            self._f("destroy")(C.c_void_p(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def advance(self, n):
+        err = C.create_string_buffer(512)
+        ok = self._f("advance")(C.c_void_p(self.h), int(n), err, 512)
+        return bool(ok), err.value.decode()
+
+    @property
+    def step(self):
+        return self._f("step")(C.c_void_p(self.h))
+
+    @property
+    def total(self):
+        return self._f("total")(C.c_void_p(self.h))
+
+    @property
+    def dt(self):
+        return self._f("dt")(C.c_void_p(self.h))
+
+    def layout(self):
+        o = np.zeros(5, np.int64)
+        self._f("layout")(C.c_void_p(self.h), o.ctypes.data_as(C.c_void_p))
+        return dict(zip(["nx", "ny", "nz", "receivers", "records"], (int(v) for v in o)))
+
+    def count(self):
+        L = self.layout()
+        return L["nx"] * L["ny"] * L["nz"]
+
+    def diagnostics(self):
+        o = np.zeros(len(DIAG3D))
+        self._f("diagnostics")(C.c_void_p(self.h), o.ctypes.data_as(C.c_void_p))
+        return dict(zip(DIAG3D, o.tolist()))
+
+    def receivers(self):
+        L = self.layout()
+        nr, n = L["receivers"], L["records"]
+        vx, vy, vz = np.zeros((nr, n)), np.zeros((nr, n)), np.zeros((nr, n))
+        pos, idx = np.zeros((nr, 3)), np.zeros(nr, np.int64)
+        self._f("receivers")(C.c_void_p(self.h), *(a.ctypes.data_as(C.c_void_p) for a in (vx, vy, vz, pos, idx)))
+        return {"vx": vx, "vy": vy, "vz": vz, "position": pos, "index": idx}
+
+    def field(self, f):
+        o = np.zeros(self.count())
+        self._f("field")(C.c_void_p(self.h), int(f), o.ctypes.data_as(C.c_void_p))
+        return o
+
+    def model(self, which):
+        n = self.count()
+        size = {0: 9 * n, 2: 3 * n, 3: 3 * n}.get(which, n)
+        o = np.zeros(size)
+        ok = self._f("model")(C.c_void_p(self.h), int(which), o.ctypes.data_as(C.c_void_p))
+        return o if ok else None
