@@ -238,6 +238,92 @@ int64_t swl_pml_slot2d(int nx_pad, int nz_pad, int width_x, int width_z, int64_t
 size_t swl_params2d_size(void);
 size_t swl_params3d_size(void);
 
+/* ---------------- 3D engine (replaces CudaEngine3D, src/three_d/CudaEngine3D.hpp:7-24) ---------------- */
+/* FP64 host mesh in padded storage q = i + nx*(j + ny*k) (Point::Mesh, src/three_d/PointKernels3D.hpp:13-21).
+   Material arrays are per node (the homogeneous case repeats the constants).  The optional
+   separable forms let the engine stream less; each is checked against the full arrays when the
+   engine is built and ignored if any value differs. */
+typedef struct swl_mesh3d_desc {
+    int precision;                   /* 0 FP64, 1 FP32 */
+    int nx, ny, nz;                  /* padded node counts (physical + 2*3) */
+    double h;
+    const double *metrics;           /* 9 per node: grad(xi), grad(eta), grad(zeta) */
+    const double *jacobian, *normals; /* normals: 3 per node, set on the surface plane k = 3 */
+    const double *lambda, *mu, *density;
+    double density_global;           /* SimulationParameters::density (source scaling, PointKernels3D.hpp:107) */
+    const double *damping, *source;
+    const double *source_gradient[3]; /* all null for a directional force */
+    double source_direction[3];
+    int flat_from;                   /* metrics are the identity and J = 1 for k >= flat_from */
+    const double *damping_axis[3];   /* damping = (X(i) + Y(j)) + Z(k), or null */
+    int material_by_depth;           /* lambda, mu, density depend on k only */
+} swl_mesh3d_desc;
+typedef struct swl_engine3d swl_engine3d;
+/* CudaEngine3D(const Point::Mesh&, receivers, device), CudaEngine3D.cu:80-103 */
+int swl_engine3d_create(const swl_mesh3d_desc *mesh, const int64_t *receiver_index, int receivers, int device,
+                        swl_engine3d **out);
+void swl_engine3d_destroy(swl_engine3d *e);
+/* CudaEngine3D::step(time, dt, frequency, step), CudaEngine3D.cu:113-124; scale = amplitude/(rho h^3) */
+int swl_engine3d_step(swl_engine3d *e, double time, double dt, double frequency, double scale, int step);
+/* rollback / resetFields, CudaEngine3D.cu:125-126 */
+int swl_engine3d_rollback(swl_engine3d *e);
+int swl_engine3d_reset(swl_engine3d *e);
+/* CudaEngine3D::diagnostics, CudaEngine3D.cu:127-133: out[10] = energy, max |V|, traction-rate ratio,
+   finite, display peaks |Vz|, |Vx|, |Vy|, |V|, |mean stress|, 0 */
+int swl_engine3d_diagnostics(swl_engine3d *e, double *out);
+/* CudaEngine3D::sample, CudaEngine3D.cu:134-138: out[3r..3r+2] = Vx, Vy, Vz */
+int swl_engine3d_sample(swl_engine3d *e, double *out);
+/* CudaEngine3D::slices, CudaEngine3D.cu:139-147: planes XZ at y, YZ at x, XY at z of `component`
+   (0 Vz, 1 Vx, 2 Vy, 3 |V|, 4 pressure); out holds nx*nz + ny*nz + nx*ny floats (physical sizes). */
+int swl_engine3d_slices(swl_engine3d *e, int x, int y, int z, int component, float *out);
+/* CudaEngine3D::download, CudaEngine3D.cu:148-150: field f (0..8) of the accepted state, padded storage. */
+int swl_engine3d_download(swl_engine3d *e, int field, double *out);
+uint64_t swl_engine3d_resident_bytes(swl_engine3d *e);
+/* Device-resident stepping (no reference counterpart; same semantics as CpuSimulation3D::advance,
+   CurvilinearSolver.cpp:445-513): up to `count` steps from `first_step`; after each the inspection,
+   the source-window peak update for (s+1)*dt <= 3/f, the fault rule
+       !finite || ((s+1)*dt > 3/f && source_peak > 0 && E > 1e4*source_peak)
+   and the receiver gather.  Per accepted step s: diag[10s..10s+9] as swl_engine3d_diagnostics,
+   samples[3*receivers*s + 3r + {0,1,2}] = Vx, Vy, Vz. */
+typedef struct swl_run3d {
+    int accepted, faulted, nonfinite; /* nonfinite: the rejected step failed the finiteness check */
+    double peak_energy, source_peak;
+} swl_run3d;
+int swl_engine3d_run(swl_engine3d *e, int first_step, int count, double dt, double frequency, double scale,
+                     double source_peak, double *diag, double *samples, swl_run3d *result);
+/* HIP-event timing of the stage kernels (one event pair per step), as for 2D. out[6]: stage ms,
+   stage launches, nodes per launch, VGPRs, metric planes stored, material by depth (0/1). */
+int swl_engine3d_profile(swl_engine3d *e, int enable);
+int swl_engine3d_kernel_timing(swl_engine3d *e, double *out);
+
+/* ---------------- 3D session (ISimulationBackend3D, include/Seis3D/CurvilinearSolver.hpp:165-185) ---------------- */
+typedef struct swl_sim3d swl_sim3d;
+/* backend 2 = HIP; precision 0 FP64 (the reference's), 1 FP32 state with FP64 closure/projection. */
+int swl_sim3d_create(const swl_params3d *p, int backend, int device, int precision, swl_sim3d **out);
+/* FP64 model only, no device (for checking the construction). */
+int swl_sim3d_plan(const swl_params3d *p, swl_sim3d **out);
+void swl_sim3d_destroy(swl_sim3d *s);
+int swl_sim3d_advance(swl_sim3d *s, int count);
+int swl_sim3d_reset(swl_sim3d *s);
+int swl_sim3d_step(const swl_sim3d *s);
+int swl_sim3d_total_steps(const swl_sim3d *s);
+double swl_sim3d_dt(const swl_sim3d *s);
+/* layout: nx_pad, ny_pad, nz_pad, receivers, samples recorded so far (t = 0 included) */
+int swl_sim3d_layout(const swl_sim3d *s, int64_t *out);
+/* diagnostics: the 18 values of the oracle order (energy, peak, source peak, max |V|, traction-rate
+   ratio, finite, min J, max metric norm, min cell length, max edge, wave/damping/source dt,
+   display peaks x5) followed by initialization ms, total step ms, resident device bytes. */
+int swl_sim3d_diagnostics(const swl_sim3d *s, double *out);
+int swl_sim3d_receivers(const swl_sim3d *s, double *vx, double *vy, double *vz, double *position, int64_t *index);
+int swl_sim3d_snapshot(const swl_sim3d *s, int field, double *out);
+/* 0 metrics(9N), 1 J, 2 normals(3N), 3 coordinates(3N), 4 lambda, 5 mu, 6 density, 7 damping, 8 source,
+   9..11 moment gradient (FP64 construction values, same numbering as the oracle) */
+int swl_sim3d_model(const swl_sim3d *s, int which, double *out);
+/* readDisplayFrame (CurvilinearSolver.cpp:545-576): component 0..4 */
+int swl_sim3d_read_display_frame(const swl_sim3d *s, int x, int y, int z, int component, float *out);
+int swl_sim3d_profile(swl_sim3d *s, int enable);
+int swl_sim3d_kernel_timing(const swl_sim3d *s, double *out);
+
 #ifdef __cplusplus
 }
 #endif

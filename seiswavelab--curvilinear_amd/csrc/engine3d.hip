@@ -1,0 +1,425 @@
+// engine3d.hip — gfx950 HIP engine for the 3D nine-field elastic curvilinear solver.
+//
+// Replaces the reference's CudaEngine3D (src/three_d/CudaEngine3D.hpp:7-24, .cu:68-150).  The per-node
+// numerics follow src/three_d/PointKernels3D.hpp (rhs :92-112, surfaceDerivative :52-66,
+// surfaceMomentum :67-91, project :36-51, energyDensity :113-120) in the same floating-point order,
+// so FP64 runs reproduce the reference's CPU path (CpuSimulation3D::advance,
+// src/three_d/CurvilinearSolver.cpp:445-513) to rounding.  Execution model:
+//   * one fused kernel per RK stage: RHS + stage combine + surface projection; the last stage also
+//     evaluates the energy / peaks / finiteness and the traction-rate residual from k4;
+//   * three rotating state buffers and one accumulator.  The accumulator holds the partial sums of
+//     base + dt/6 k1 + dt/3 k2 + dt/3 k3 in the reference's left-to-right order (CurvilinearSolver.cpp:472),
+//     so the last stage reads the accumulator and its input only, and rollback is a buffer index;
+//   * aligned layout (node (i,j,k) at off + i + ld*(j + ny*k), column 3 on a 64-element boundary);
+//   * metrics and Jacobians are stored only for the terrain band k < flat_from (identity below, checked
+//     when the engine is built); layered materials depending on k only and the separable sponge
+//     rate (X(i) + Y(j)) + Z(k) are read from 1D tables;
+//   * the surface closure and projection run in FP64 also in the FP32 variant;
+//   * device-side fault rule with the source-window peak and receiver gather (swl_engine3d_run).
+#include "common.hpp"
+#include "swl.h"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+namespace swl {
+namespace {
+
+constexpr int TX = 64, TY = 4, THREADS = TX * TY;
+constexpr int ORIGIN = 61; // off: column 3 lands on a 64-element boundary
+
+template <class R> struct DMesh3 {
+    int nx, ny, nz, ld, off, flat;
+    long long plane; // ld * ny
+    R h, hinv;       // hinv = 1/h when h is a power of two (exact), else 0
+    const R *met;    // 9 arrays of mcount elements (k < flat)
+    const R *jac;    // mcount (k < flat)
+    long long mcount;
+    int by_depth;
+    const R *lam, *mu, *rho; // per node, or per k when by_depth
+    const R *dax, *day, *daz, *damp; // separable sponge tables, or full array
+    R rho_global, dir[3];
+    const R *src, *grad[3]; // full arrays, read inside the source box only
+    int si0, si1, sj0, sj1, sk0, sk1;
+    // FP64 surface plane (k = 3), indexed by i + ld*j
+    const double *smet, *slam, *smu, *snrm;
+};
+template <class R> __device__ __host__ inline long long at3(const DMesh3<R> &m, int i, int j, int k) {
+    return (long long)m.off + i + (long long)m.ld * j + m.plane * k;
+}
+
+struct Partial3 {
+    double energy, velocity, peak[5], traction, norm;
+    int bad, pad;
+};
+__device__ inline void merge3(Partial3 &a, const Partial3 &b) {
+    a.energy += b.energy;
+    a.velocity = fmax(a.velocity, b.velocity);
+    for (int c = 0; c < 5; ++c) a.peak[c] = fmax(a.peak[c], b.peak[c]);
+    a.traction = fmax(a.traction, b.traction);
+    a.norm = fmax(a.norm, b.norm);
+    a.bad |= b.bad;
+}
+__device__ void block_reduce3(Partial3 &p, Partial3 *sh) {
+    const int t = threadIdx.x;
+    sh[t] = p;
+    __syncthreads();
+    for (int s = THREADS / 2; s > 0; s >>= 1) {
+        if (t < s) merge3(sh[t], sh[t + s]);
+        __syncthreads();
+    }
+}
+
+template <class R> struct Stage3 {
+    DMesh3<R> m;
+    const R *in, *base;
+    R *out, *acc;
+    size_t count;
+    R forcing, w, wy; // stage weight and accumulator weight (dt/6, dt/3, dt/3, dt/6)
+    int kind;         // 0 first, 1 middle, 2 final (+ inspection)
+    const int *status;
+    Partial3 *partials;
+    int ntx, nty, ntiles;
+};
+
+__device__ inline int tile_of_block(int b, int n) { // XCD-contiguous band of tiles
+    const int x = b & 7, j = b >> 3, q = n >> 3, r = n & 7;
+    return x * q + min(x, r) + j;
+}
+
+// ---- MacCormack-DRP operators (FiniteDifference.hpp:14-25)
+template <class R, int DIR> __device__ inline R cf(int l) {
+    const R c = l == 0 ? (R)-.30874 : l == 1 ? (R)-.6326 : l == 2 ? (R)1.233 : l == 3 ? (R)-.3334 : (R).04168;
+    return DIR ? -c : c;
+}
+template <int DIR> __device__ constexpr int of(int l) { return DIR ? -(l - 1) : (l - 1); }
+template <class R> __device__ inline R byh(R r, R h, R hinv) { return hinv != (R)0 ? r * hinv : r / h; }
+template <class R, int DIR> __device__ inline R drp(const R *__restrict__ v, long long q, long long s, R h, R hinv) {
+    R r = 0;
+#pragma unroll
+    for (int l = 0; l < 5; ++l) r += cf<R, DIR>(l) * v[q + of<DIR>(l) * s];
+    return byh(r, h, hinv);
+}
+// reduced one-sided stencils for velocity z-derivatives at depth 1 (2-pt) and 2 (3-pt)
+template <class R, int DIR> __device__ inline R low3(const R *__restrict__ v, long long q, long long s, R h, R hinv) {
+    const R l0 = ((R)-7) / ((R)6), l1 = ((R)8) / ((R)6), l2 = ((R)-1) / ((R)6);
+    const R c0 = DIR ? -l0 : l0, c1 = DIR ? -l1 : l1, c2 = DIR ? -l2 : l2;
+    constexpr int o1 = DIR ? -1 : 1, o2 = DIR ? -2 : 2;
+    R r = 0;
+    r += c0 * v[q];
+    r += c1 * v[q + o1 * s];
+    r += c2 * v[q + o2 * s];
+    return byh(r, h, hinv);
+}
+template <class R, int DIR> __device__ inline R low2(const R *__restrict__ v, long long q, long long s, R h, R hinv) {
+    const R c0 = DIR ? (R)1 : (R)-1, c1 = DIR ? (R)-1 : (R)1;
+    constexpr int o1 = DIR ? -1 : 1;
+    R r = 0;
+    r += c0 * v[q];
+    r += c1 * v[q + o1 * s];
+    return byh(r, h, hinv);
+}
+
+template <class R> __device__ inline R metric(const DMesh3<R> &m, long long q, int k, int a) {
+    return k < m.flat ? m.met[(size_t)a * m.mcount + q] : (a % 4 == 0 ? (R)1 : (R)0);
+}
+template <class R> __device__ inline R jacobian(const DMesh3<R> &m, long long q, int k) {
+    return k < m.flat ? m.jac[q] : (R)1;
+}
+
+// traction flux J*(grad axis).sigma (PointKernels3D.hpp:67-72)
+template <class R>
+__device__ inline R tflux(const DMesh3<R> &m, const R *__restrict__ w, size_t count, long long q, int k, int axis, int comp) {
+    const int f0 = comp == 0 ? 3 : comp == 1 ? 6 : 7, f1 = comp == 0 ? 6 : comp == 1 ? 4 : 8, f2 = comp == 0 ? 7 : comp == 1 ? 8 : 5;
+    R v = 0;
+    v += metric(m, q, k, 3 * axis + 0) * w[(size_t)f0 * count + q];
+    v += metric(m, q, k, 3 * axis + 1) * w[(size_t)f1 * count + q];
+    v += metric(m, q, k, 3 * axis + 2) * w[(size_t)f2 * count + q];
+    return jacobian(m, q, k) * v;
+}
+// surfaceMomentum (PointKernels3D.hpp:73-91)
+template <class R, int DX, int DY, int DZ>
+__device__ R smomentum(const DMesh3<R> &m, const R *__restrict__ w, size_t count, long long q, int k, int comp, int depth,
+                       R rho) {
+    R sum = 0;
+#pragma unroll
+    for (int l = 0; l < 5; ++l) sum += cf<R, DX>(l) * tflux(m, w, count, q + of<DX>(l), k, 0, comp);
+#pragma unroll
+    for (int l = 0; l < 5; ++l) sum += cf<R, DY>(l) * tflux(m, w, count, q + of<DY>(l) * (long long)m.ld, k, 1, comp);
+#pragma unroll
+    for (int l = 0; l < 5; ++l) {
+        int o = of<DZ>(l);
+        R sign = 1;
+        const int z = depth + o;
+        if (z == 0) continue;
+        if (z < 0) o = -z - depth, sign = -1;
+        sum += sign * cf<R, DZ>(l) * tflux(m, w, count, q + o * m.plane, k + o, 2, comp);
+    }
+    return sum / (m.h * jacobian(m, q, k) * rho);
+}
+
+// surfaceDerivative (PointKernels3D.hpp:52-66), FP64
+template <class R, int DX, int DY>
+__device__ void closure(const DMesh3<R> &m, const R *__restrict__ w, size_t count, long long q, int i, int j, double *res) {
+    const long long sq = i + (long long)m.ld * j;
+    const double *mt = m.smet + 9 * sq;
+    const double lambda = m.slam[sq], mu = m.smu[sq];
+    const double c[3] = {mt[6], mt[7], mt[8]};
+    const double c2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+    const double length = sqrt(c2), n[3] = {c[0] / length, c[1] / length, c[2] / length};
+    const double h = (double)m.h;
+    double g[9], tr[3] = {0, 0, 0};
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        const R *f = w + (size_t)v * count;
+        double dx = 0, dy = 0;
+#pragma unroll
+        for (int l = 0; l < 5; ++l) dx += cf<double, DX>(l) * (double)f[q + of<DX>(l)];
+#pragma unroll
+        for (int l = 0; l < 5; ++l) dy += cf<double, DY>(l) * (double)f[q + of<DY>(l) * (long long)m.ld];
+        dx = dx / h, dy = dy / h;
+        for (int a = 0; a < 3; ++a) g[3 * v + a] = mt[a] * dx + mt[3 + a] * dy;
+    }
+    const double trace = g[0] + g[4] + g[8];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) tr[a] += ((a == b ? lambda * trace : 0) + mu * (g[3 * a + b] + g[3 * b + a])) * c[b];
+    const double normal = tr[0] * n[0] + tr[1] * n[1] + tr[2] * n[2];
+    for (int a = 0; a < 3; ++a) res[a] = -(tr[a] - n[a] * normal) / (mu * c2) - n[a] * normal / ((lambda + 2 * mu) * c2);
+}
+
+// project (PointKernels3D.hpp:36-51), FP64, on the six stresses of one surface node
+__device__ inline void project9(const double *n, double *s6 /* xx yy zz xy xz yz */) {
+    const double sg[9] = {s6[0], s6[3], s6[4], s6[3], s6[1], s6[5], s6[4], s6[5], s6[2]};
+    double t[3] = {0, 0, 0}, o[9];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) t[a] += sg[3 * a + b] * n[b];
+    const double nn = t[0] * n[0] + t[1] * n[1] + t[2] * n[2];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) o[3 * a + b] = sg[3 * a + b] - n[a] * t[b] - t[a] * n[b] + nn * n[a] * n[b];
+    s6[0] = o[0], s6[1] = o[4], s6[2] = o[8];
+    s6[3] = 0.5 * (o[1] + o[3]), s6[4] = 0.5 * (o[2] + o[6]), s6[5] = 0.5 * (o[5] + o[7]);
+}
+
+template <class R> __device__ inline R mat(const DMesh3<R> &m, const R *a, long long q, int k) {
+    return m.by_depth ? a[k] : a[q];
+}
+
+// One RK stage over every physical node (one node per lane, 64 x 4 nodes per block, one k plane).
+template <class R, int DX, int DY, int DZ> __global__ __launch_bounds__(THREADS) void stage3_kernel(Stage3<R> A) {
+    __shared__ Partial3 sh[THREADS];
+    if (*A.status) return;
+    const DMesh3<R> &m = A.m;
+    const int t = tile_of_block(blockIdx.x, A.ntiles);
+    const int per = A.ntx * A.nty, kk = t / per, rem = t % per;
+    const int i = 3 + (rem % A.ntx) * TX + (threadIdx.x % TX), j = 3 + (rem / A.ntx) * TY + threadIdx.x / TX, k = 3 + kk;
+    Partial3 p{};
+    if (i < m.nx - 3 && j < m.ny - 3) {
+        const size_t cnt = A.count;
+        const long long q = at3(m, i, j, k), sy = m.ld, sz = m.plane;
+        const R *w = A.in;
+        const R h = m.h, hinv = m.hinv;
+        const int depth = k - 3;
+        const R lambda = mat(m, m.lam, q, k), mu = mat(m, m.mu, q, k), density = mat(m, m.rho, q, k);
+        double dzs[3] = {0, 0, 0};
+        if (depth == 0) closure<R, DX, DY>(m, w, cnt, q, i, j, dzs);
+        R grad[9][3];
+        R mt[9];
+#pragma unroll
+        for (int a = 0; a < 9; ++a) mt[a] = metric(m, q, k, a);
+#pragma unroll
+        for (int f = 0; f < 9; ++f) {
+            const R *v = w + (size_t)f * cnt;
+            R d[3];
+            d[0] = drp<R, DX>(v, q, 1, h, hinv);
+            d[1] = drp<R, DY>(v, q, sy, h, hinv);
+            if (f < 3 && depth == 0) d[2] = (R)dzs[f];
+            else if (f < 3 && depth == 1) d[2] = low2<R, DZ>(v, q, sz, h, hinv);
+            else if (f < 3 && depth == 2) d[2] = low3<R, DZ>(v, q, sz, h, hinv);
+            else d[2] = drp<R, DZ>(v, q, sz, h, hinv);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) grad[f][a] = mt[a] * d[0] + mt[3 + a] * d[1] + mt[6 + a] * d[2];
+        }
+        R kv[9];
+        kv[0] = (grad[3][0] + grad[6][1] + grad[7][2]) / density;
+        kv[1] = (grad[6][0] + grad[4][1] + grad[8][2]) / density;
+        kv[2] = (grad[7][0] + grad[8][1] + grad[5][2]) / density;
+        if (depth <= (DZ ? 3 : 1))
+            for (int a = 0; a < 3; ++a) kv[a] = smomentum<R, DX, DY, DZ>(m, w, cnt, q, k, a, depth, density);
+        if (m.si1 >= 0 && i >= m.si0 && i <= m.si1 && j >= m.sj0 && j <= m.sj1 && k >= m.sk0 && k <= m.sk1) {
+            for (int a = 0; a < 3; ++a)
+                kv[a] += A.forcing * (m.grad[0] ? m.grad[a][q] : m.src[q] * m.dir[a]) * (m.rho_global / density);
+        }
+        const R div = grad[0][0] + grad[1][1] + grad[2][2];
+        kv[3] = lambda * div + 2 * mu * grad[0][0];
+        kv[4] = lambda * div + 2 * mu * grad[1][1];
+        kv[5] = lambda * div + 2 * mu * grad[2][2];
+        kv[6] = mu * (grad[0][1] + grad[1][0]);
+        kv[7] = mu * (grad[0][2] + grad[2][0]);
+        kv[8] = mu * (grad[1][2] + grad[2][1]);
+        const R damp = m.damp ? m.damp[q] : (m.dax[i] + m.day[j]) + m.daz[k];
+#pragma unroll
+        for (int f = 0; f < 9; ++f) kv[f] -= damp * w[(size_t)f * cnt + q];
+        // combine (CurvilinearSolver.cpp:466-472) with the accumulator partial sums
+        R nxt[9];
+#pragma unroll
+        for (int f = 0; f < 9; ++f) {
+            const size_t o = (size_t)f * cnt + q;
+            if (A.kind == 2) {
+                nxt[f] = A.acc[o] + A.wy * kv[f];
+            } else {
+                const R b = A.base[o];
+                nxt[f] = b + A.w * kv[f];
+                A.acc[o] = (A.kind == 0 ? b : A.acc[o]) + A.wy * kv[f];
+            }
+        }
+        if (depth == 0) { // projection on the surface plane, FP64
+            const double *n = m.snrm + 3 * (i + (long long)m.ld * j);
+            double s6[6];
+            for (int f = 0; f < 6; ++f) s6[f] = (double)nxt[3 + f];
+            project9(n, s6);
+            for (int f = 0; f < 6; ++f) nxt[3 + f] = (R)s6[f];
+        }
+#pragma unroll
+        for (int f = 0; f < 9; ++f) A.out[(size_t)f * cnt + q] = nxt[f];
+        if (A.kind == 2) {
+            // Impl::updateDiagnostics (CurvilinearSolver.cpp:226-245) on the new state, rate residual from k4
+            bool bad = false;
+            for (int f = 0; f < 9; ++f) bad = bad || !isfinite(nxt[f]);
+            const double vx = nxt[0], vy = nxt[1], vz = nxt[2];
+            const double v2 = vx * vx + vy * vy + vz * vz, sv = sqrt(v2);
+            p.velocity = sv;
+            p.peak[0] = fabs(vz), p.peak[1] = fabs(vx), p.peak[2] = fabs(vy), p.peak[3] = sv;
+            const double sxx = nxt[3], syy = nxt[4], szz = nxt[5], sxy = nxt[6], sxz = nxt[7], syz = nxt[8];
+            p.peak[4] = fabs((sxx + syy + szz) / 3.0);
+            const double dl = lambda, dm = mu, dr = density;
+            const double mean = (sxx + syy + szz) / 3;
+            double dev = 0;
+            const double tt[3] = {sxx - mean, syy - mean, szz - mean};
+            for (int c = 0; c < 3; ++c) dev += tt[c] * tt[c];
+            dev += 2 * sxy * sxy, dev += 2 * sxz * sxz, dev += 2 * syz * syz;
+            const double bulk = dl + 2 * dm / 3;
+            const double hh = (double)h;
+            p.energy = (0.5 * dr * v2 + dev / (4 * dm) + mean * mean / (2 * bulk)) * (double)jacobian(m, q, k) * hh * hh * hh;
+            if (!isfinite(v2) || !isfinite(p.energy)) bad = true;
+            if (depth == 0) {
+                const double *n = m.snrm + 3 * (i + (long long)m.ld * j);
+                const double t3[3] = {(double)kv[3] * n[0] + (double)kv[6] * n[1] + (double)kv[7] * n[2],
+                                      (double)kv[6] * n[0] + (double)kv[4] * n[1] + (double)kv[8] * n[2],
+                                      (double)kv[7] * n[0] + (double)kv[8] * n[1] + (double)kv[5] * n[2]};
+                p.traction = sqrt(t3[0] * t3[0] + t3[1] * t3[1] + t3[2] * t3[2]);
+                double norm2 = 0.0;
+                for (int f = 3; f < 9; ++f) norm2 += (f >= 6 ? 2.0 : 1.0) * (double)kv[f] * (double)kv[f];
+                p.norm = sqrt(norm2);
+            }
+            p.bad = bad;
+        }
+    }
+    if (A.kind == 2) {
+        block_reduce3(p, sh);
+        if (threadIdx.x == 0) A.partials[t] = sh[0];
+    }
+}
+
+__global__ __launch_bounds__(THREADS) void reduce3_kernel(const Partial3 *partials, int n, Partial3 *level2,
+                                                          const int *status) {
+    __shared__ Partial3 sh[THREADS];
+    if (status && *status) return;
+    const int per = (n + gridDim.x - 1) / gridDim.x, begin = blockIdx.x * per, end = min(n, begin + per);
+    Partial3 p{};
+    for (int t = begin + threadIdx.x; t < end; t += THREADS) merge3(p, partials[t]);
+    block_reduce3(p, sh);
+    if (threadIdx.x == 0) level2[blockIdx.x] = sh[0];
+}
+
+template <class R> struct Final3 {
+    const Partial3 *partials;
+    int n;
+    int *status, *accepted;
+    double *peak, *source_peak;
+    double *diag, *samples, *last; // last: diagnostics of the latest evaluated step (10 values)
+    const R *state;
+    const long long *probes;
+    int nrec, row, step, rule; // rule 0: always accept (single-step API)
+    double dt, frequency;
+};
+__device__ inline void diag_row(const Partial3 &r, double *d) {
+    const double ratio = r.traction / fmax(r.norm, 1.0e-30);
+    const bool finite = !r.bad && isfinite(r.energy) && isfinite(ratio);
+    d[0] = r.energy, d[1] = r.velocity, d[2] = ratio, d[3] = finite;
+    for (int c = 0; c < 5; ++c) d[4 + c] = r.peak[c];
+    d[9] = 0;
+}
+template <class R> __global__ __launch_bounds__(THREADS) void final3_kernel(Final3<R> A) {
+    __shared__ Partial3 sh[THREADS];
+    __shared__ int verdict;
+    if (*A.status) return;
+    Partial3 p{};
+    for (int t = threadIdx.x; t < A.n; t += THREADS) merge3(p, A.partials[t]);
+    block_reduce3(p, sh);
+    if (threadIdx.x == 0) {
+        double d[10];
+        diag_row(sh[0], d);
+        for (int c = 0; c < 10; ++c) A.last[c] = d[c];
+        const double E = d[0];
+        bool fault = false;
+        if (A.rule) {
+            const double t = (A.step + 1) * A.dt, window = 3.0 / A.frequency;
+            if (t <= window) *A.source_peak = fmax(*A.source_peak, E);
+            const double sp = *A.source_peak;
+            const bool growth = t > window && sp > 0 && E > 1e4 * sp;
+            fault = d[3] == 0 || growth;
+        }
+        if (fault) {
+            *A.status = 1;
+        } else {
+            *A.peak = fmax(*A.peak, E);
+            if (A.diag)
+                for (int c = 0; c < 10; ++c) A.diag[10 * (size_t)A.row + c] = d[c];
+            *A.accepted = A.row + 1;
+        }
+        verdict = fault;
+    }
+    __syncthreads();
+    if (verdict || !A.samples) return;
+    for (int r = threadIdx.x; r < A.nrec; r += THREADS) {
+        double *o = A.samples + (size_t)A.row * 3 * A.nrec + 3 * r;
+        for (int c = 0; c < 3; ++c) o[c] = A.state[A.probes[3 * r + c]];
+    }
+}
+
+template <class R> __global__ void gather3_kernel(const R *state, const long long *probes, int n, double *out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n)
+        for (int c = 0; c < 3; ++c) out[3 * r + c] = state[probes[3 * r + c]];
+}
+// readDisplayFrame / slices (CudaEngine3D.cu:59-66): plane 0 XZ at y, 1 YZ at x, 2 XY at z
+template <class R>
+__global__ void slice3_kernel(const R *state, size_t count, DMesh3<R> m, int x, int y, int z, int comp, int plane,
+                              int width, int height, float *out) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= width * height) return;
+    const int a = q % width, b = q / width;
+    const int i = (plane == 1 ? x : a) + 3, j = (plane == 0 ? y : plane == 1 ? a : b) + 3, k = (plane == 2 ? z : b) + 3;
+    const long long p = at3(m, i, j, k);
+    const double vx = state[p], vy = state[count + p], vz = state[2 * count + p];
+    const double value = comp == 0   ? vz
+                         : comp == 1 ? vx
+                         : comp == 2 ? vy
+                         : comp == 4 ? -((double)state[3 * count + p] + state[4 * count + p] + state[5 * count + p]) / 3
+                                     : sqrt(vx * vx + vy * vy + vz * vz);
+    out[q] = (float)value;
+}
+template <class R> __global__ void plain3_kernel(const R *field, DMesh3<R> m, double *out) {
+    const long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const long long n = (long long)m.nx * m.ny * m.nz;
+    if (q >= n) return;
+    const int i = (int)(q % m.nx), j = (int)((q / m.nx) % m.ny), k = (int)(q / ((long long)m.nx * m.ny));
+    out[q] = (double)field[at3(m, i, j, k)];
+}
+
+} // namespace
+} // namespace swl
+
+#include "engine3d_host.inl"

@@ -23,6 +23,11 @@ class Run2D(C.Structure):
     _fields_ = [("accepted", C.c_int), ("faulted", C.c_int), ("peak_energy", C.c_double)]
 
 
+class Run3D(C.Structure):
+    _fields_ = [("accepted", C.c_int), ("faulted", C.c_int), ("nonfinite", C.c_int), ("peak_energy", C.c_double),
+                ("source_peak", C.c_double)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "swl_last_error": (C.c_char_p, []),
@@ -71,6 +76,36 @@ SIGNATURES = {
     "swl_pml_slot2d": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int]),
     "swl_params2d_size": (C.c_size_t, []),
     "swl_params3d_size": (C.c_size_t, []),
+    "swl_engine3d_create": (C.c_int, [vp, vp, C.c_int, C.c_int, P(vp)]),
+    "swl_engine3d_destroy": (None, [vp]),
+    "swl_engine3d_step": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]),
+    "swl_engine3d_rollback": (C.c_int, [vp]),
+    "swl_engine3d_reset": (C.c_int, [vp]),
+    "swl_engine3d_diagnostics": (C.c_int, [vp, vp]),
+    "swl_engine3d_sample": (C.c_int, [vp, vp]),
+    "swl_engine3d_slices": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]),
+    "swl_engine3d_download": (C.c_int, [vp, C.c_int, vp]),
+    "swl_engine3d_resident_bytes": (C.c_uint64, [vp]),
+    "swl_engine3d_run": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, vp, vp,
+                                   P(Run3D)]),
+    "swl_engine3d_profile": (C.c_int, [vp, C.c_int]),
+    "swl_engine3d_kernel_timing": (C.c_int, [vp, vp]),
+    "swl_sim3d_create": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, P(vp)]),
+    "swl_sim3d_plan": (C.c_int, [vp, P(vp)]),
+    "swl_sim3d_destroy": (None, [vp]),
+    "swl_sim3d_advance": (C.c_int, [vp, C.c_int]),
+    "swl_sim3d_reset": (C.c_int, [vp]),
+    "swl_sim3d_step": (C.c_int, [vp]),
+    "swl_sim3d_total_steps": (C.c_int, [vp]),
+    "swl_sim3d_dt": (C.c_double, [vp]),
+    "swl_sim3d_layout": (C.c_int, [vp, vp]),
+    "swl_sim3d_diagnostics": (C.c_int, [vp, vp]),
+    "swl_sim3d_receivers": (C.c_int, [vp, vp, vp, vp, vp, vp]),
+    "swl_sim3d_snapshot": (C.c_int, [vp, C.c_int, vp]),
+    "swl_sim3d_model": (C.c_int, [vp, C.c_int, vp]),
+    "swl_sim3d_read_display_frame": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]),
+    "swl_sim3d_profile": (C.c_int, [vp, C.c_int]),
+    "swl_sim3d_kernel_timing": (C.c_int, [vp, vp]),
 }
 
 

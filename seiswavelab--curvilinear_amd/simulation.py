@@ -213,3 +213,149 @@ def writeReceivers(sim: Simulation, path: str):
     """Receiver CSV with the reference's provenance header (CurvilinearSolver2D.cpp:811-854)."""
     rc = native.lib().swl_sim2d_write_receivers(sim._h, str(path).encode())
     return (rc == 0), ("" if rc == 0 else native.last_error())
+
+
+# ---------------------------------------------------------------------------------------------
+# 3D: mirror of ISimulationBackend3D / CpuSimulation3D (include/Seis3D/CurvilinearSolver.hpp:165-218)
+
+DIAGNOSTIC_NAMES_3D = ["elasticEnergy", "peakEnergy", "sourcePeakEnergy", "maxAbsVelocity", "maxTractionRateRelative",
+                       "allFinite", "minJacobian", "maxMetricNorm", "minCellLength", "maxCellEdge", "waveDtLimit",
+                       "dampingDtLimit", "sourceDtLimit", "peakVz", "peakVx", "peakVy", "peakV", "peakPressure",
+                       "initializationMs", "totalStepMs", "residentDeviceBytes"]
+MODEL_ARRAYS_3D = {"metrics": 0, "jacobian": 1, "normals": 2, "coordinates": 3, "lambda": 4, "mu": 5, "density": 6,
+                   "damping": 7, "source": 8, "gradientX": 9, "gradientY": 10, "gradientZ": 11}
+
+
+@dataclass
+class ReceiverTrace3D:
+    position: Tuple[float, float, float]
+    gridIndex: int
+    samples: np.ndarray  # Vz (the reference keeps raw Vz in `samples`)
+    samplesVx: np.ndarray
+    samplesVy: np.ndarray
+
+
+class Simulation3D:
+    """3D nine-field elastic session on the HIP backend (FP64, or FP32 state with FP64 closure)."""
+
+    def __init__(self, backend: BackendKind = BackendKind.Hip):
+        self._h = None
+        self._params = None
+        self._backend = backend
+
+    def initialize(self, params, device: int = 0, precision: int = 0):
+        from .params import to_c3d
+        c, keep = to_c3d(params)
+        out = C.c_void_p()
+        lib = native.lib()
+        if self._backend == BackendKind.Hip:
+            rc = lib.swl_sim3d_create(C.byref(c), int(self._backend), device, int(precision), C.byref(out))
+        elif self._backend == BackendKind.Cpu:
+            rc = lib.swl_sim3d_plan(C.byref(c), C.byref(out))  # device-free plan (no stepping)
+        else:
+            return False, "Unavailable backend: this build provides HIP (and device-free planning)."
+        if rc != 0:
+            return False, native.last_error()
+        self.close()
+        self._h, self._params = out, params
+        return True, ""
+
+    def close(self):
+        if self._h:
+            native.lib().swl_sim3d_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _need(self):
+        if not self._h:
+            raise native.NativeError("Initialize a model before stepping.")
+
+    def advance(self, count: int):
+        self._need()
+        rc = native.lib().swl_sim3d_advance(self._h, int(count))
+        return (rc == 0), ("" if rc == 0 else native.last_error())
+
+    def resetFields(self):
+        self._need()
+        rc = native.lib().swl_sim3d_reset(self._h)
+        return (rc == 0), ("" if rc == 0 else native.last_error())
+
+    def initialized(self) -> bool:
+        return bool(self._h)
+
+    def currentStep(self) -> int:
+        return native.lib().swl_sim3d_step(self._h) if self._h else 0
+
+    def totalSteps(self) -> int:
+        return native.lib().swl_sim3d_total_steps(self._h) if self._h else 0
+
+    def completed(self) -> bool:
+        return bool(self._h) and self.currentStep() >= self.totalSteps()
+
+    def dt(self) -> float:
+        return native.lib().swl_sim3d_dt(self._h) if self._h else 0.0
+
+    def parameters(self):
+        return self._params
+
+    def layout(self) -> dict:
+        o = np.zeros(5, np.int64)
+        native.lib().swl_sim3d_layout(self._h, _dp(o))
+        return dict(zip(["nx", "ny", "nz", "receivers", "records"], (int(v) for v in o)))
+
+    def diagnostics(self) -> dict:
+        o = np.zeros(len(DIAGNOSTIC_NAMES_3D))
+        native.lib().swl_sim3d_diagnostics(self._h, _dp(o))
+        d = dict(zip(DIAGNOSTIC_NAMES_3D, o.tolist()))
+        d["allFinite"] = bool(d["allFinite"])
+        return d
+
+    def receiver_arrays(self):
+        L = self.layout()
+        nr, n = L["receivers"], L["records"]
+        vx, vy, vz = np.zeros((nr, n)), np.zeros((nr, n)), np.zeros((nr, n))
+        pos, idx = np.zeros((nr, 3)), np.zeros(nr, np.int64)
+        native.lib().swl_sim3d_receivers(self._h, *(_dp(a) for a in (vx, vy, vz, pos, idx)))
+        return {"vx": vx, "vy": vy, "vz": vz, "position": pos, "index": idx}
+
+    def receivers(self) -> List[ReceiverTrace3D]:
+        a = self.receiver_arrays()
+        return [ReceiverTrace3D(tuple(a["position"][r]), int(a["index"][r]), a["vz"][r], a["vx"][r], a["vy"][r])
+                for r in range(a["vz"].shape[0])]
+
+    def field(self, component: int) -> np.ndarray:
+        L = self.layout()
+        o = np.zeros(L["nx"] * L["ny"] * L["nz"])
+        native.call("swl_sim3d_snapshot", self._h, int(component), _dp(o))
+        return o
+
+    def model_array(self, name_or_index) -> np.ndarray:
+        which = MODEL_ARRAYS_3D.get(name_or_index, name_or_index)
+        L = self.layout()
+        n = L["nx"] * L["ny"] * L["nz"]
+        o = np.zeros({0: 9 * n, 2: 3 * n, 3: 3 * n}.get(which, n))
+        native.call("swl_sim3d_model", self._h, int(which), _dp(o))
+        return o
+
+    def readDisplayFrame(self, x: int, y: int, z: int, component: int):
+        p = self._params
+        out = np.zeros(p.nx * p.nz + p.ny * p.nz + p.nx * p.ny, np.float32)
+        rc = native.lib().swl_sim3d_read_display_frame(self._h, int(x), int(y), int(z), int(component), _dp(out))
+        if rc != 0:
+            return False, native.last_error(), None
+        a, b = p.nx * p.nz, p.ny * p.nz
+        return True, "", [out[:a].reshape(p.nz, p.nx), out[a:a + b].reshape(p.nz, p.ny), out[a + b:].reshape(p.ny, p.nx)]
+
+    def profile(self, enable: bool = True):
+        native.call("swl_sim3d_profile", self._h, int(enable))
+
+    def kernel_timing(self) -> dict:
+        o = np.zeros(6)
+        native.call("swl_sim3d_kernel_timing", self._h, _dp(o))
+        return {"stage_ms": o[0], "stage_launches": int(o[1]), "nodes_per_launch": int(o[2]), "vgprs": int(o[3]),
+                "metric_planes": int(o[4]), "material_by_depth": bool(o[5])}

@@ -1,0 +1,106 @@
+"""3D HIP engine (csrc/engine3d.hip) against the CPU oracle, through the C-ABI.
+
+FP64: the engine follows the reference's operation order, so fields and traces match the oracle to
+rounding (bar: 1e-10 relative L2; the energy sum is a parallel reduction, bar 1e-12).  FP32: state
+in FP32, surface closure / projection / energy in FP64; bar 1e-4 relative L2 over the whole gather.
+"""
+import numpy as np
+import pytest
+
+from cpu_sessions import CpuSession3D, rel_l2
+from seiswavelab_curvilinear_amd import BackendKind, Simulation3D
+from test_oracle_3d import CASES, case
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu(p, precision):
+    print(f"[3d] case nx={p.nx} ny={p.ny} nz={p.nz} precision={precision}", flush=True)
+    s = Simulation3D(BackendKind.Hip)
+    ok, err = s.initialize(p, 0, precision)
+    assert ok, err
+    return s
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_fp64_matches_oracle_3d(name):
+    p = case(name)
+    o = CpuSession3D("orc", p)
+    s = _gpu(p, 0)
+    assert s.dt() == o.dt and s.totalSteps() == o.total
+    ok, err = s.advance(p.maxSteps)
+    assert ok, err
+    assert o.advance(p.maxSteps)[0]
+    ro, rs = o.receivers(), s.receiver_arrays()
+    gather_o = np.concatenate([ro["vx"], ro["vy"], ro["vz"]])
+    gather_s = np.concatenate([rs["vx"], rs["vy"], rs["vz"]])
+    assert np.abs(gather_o).max() > 0
+    assert rel_l2(gather_o, gather_s) <= 1e-10
+    for f in range(9):
+        assert rel_l2(o.field(f), s.field(f)) <= 1e-10, f
+    do, ds = o.diagnostics(), s.diagnostics()
+    assert abs(ds["elasticEnergy"] - do["energy"]) <= 1e-12 * do["energy"]
+    assert abs(ds["peakEnergy"] - do["peakEnergy"]) <= 1e-12 * do["peakEnergy"]
+    for k, kk in (("maxAbsVelocity", "maxAbsVelocity"), ("peakVz", "peakVz"), ("peakPressure", "peakPressure")):
+        assert abs(ds[k] - do[kk]) <= 1e-10 * max(abs(do[kk]), 1e-300), k
+    assert ds["allFinite"] and ds["maxTractionRateRelative"] < 1e-9
+
+
+@pytest.mark.parametrize("name", ["analytic_sponge_force", "dem_layers_moment"])
+def test_fp32_within_tolerance_of_fp64_oracle_3d(name):
+    p = case(name)
+    o = CpuSession3D("orc", p)
+    s = _gpu(p, 1)
+    ok, err = s.advance(p.maxSteps)
+    assert ok, err
+    assert o.advance(p.maxSteps)[0]
+    ro, rs = o.receivers(), s.receiver_arrays()
+    gather_o = np.concatenate([ro["vx"], ro["vy"], ro["vz"]])
+    gather_s = np.concatenate([rs["vx"], rs["vy"], rs["vz"]])
+    assert rel_l2(gather_o, gather_s) <= 1e-4
+    d = s.diagnostics()
+    assert d["allFinite"] and d["maxTractionRateRelative"] < 5e-6
+
+
+def test_reset_is_bitwise_reproducible_and_chunking_invariant_3d():
+    p = case("dem_layers_moment")
+    s = _gpu(p, 0)
+    assert s.advance(p.maxSteps)[0]
+    first = s.receiver_arrays()
+    f0 = s.field(2)
+    assert s.resetFields()[0]
+    assert s.currentStep() == 0
+    for n in (5, 1, 7, p.maxSteps - 13):
+        assert s.advance(n)[0]
+    again = s.receiver_arrays()
+    assert np.array_equal(first["vz"], again["vz"]) and np.array_equal(first["vx"], again["vx"])
+    assert np.array_equal(f0, s.field(2))
+
+
+def test_display_frame_planes_3d():
+    p = case("analytic_sponge_force")
+    s = _gpu(p, 0)
+    assert s.advance(10)[0]
+    ok, err, planes = s.readDisplayFrame(p.nx // 2, p.ny // 2, 1, 0)
+    assert ok, err
+    vz = s.field(2).reshape(p.nz + 6, p.ny + 6, p.nx + 6)
+    assert np.array_equal(planes[0], vz[3:-3, p.ny // 2 + 3, 3:-3].astype(np.float32))
+    assert np.array_equal(planes[1], vz[3:-3, 3:-3, p.nx // 2 + 3].astype(np.float32))
+    assert np.array_equal(planes[2], vz[4, 3:-3, 3:-3].astype(np.float32))
+    ok, err, _ = s.readDisplayFrame(p.nx, 0, 0, 0)
+    assert not ok and "outside the active physical grid" in err
+
+
+def test_nonfinite_model_rejects_step_and_latches_fault_3d():
+    p = case("analytic_sponge_force")
+    p.sourceForce = 1e308
+    s = _gpu(p, 0)
+    ok, err = s.advance(p.maxSteps)
+    assert not ok
+    assert err in ("Non-finite RK4 step rejected. Last accepted fields retained; rebuild with smaller CFL.",
+                   "Energy grew above 10,000 times the source-window peak. Last accepted step retained; "
+                   "rebuild with smaller CFL or gentler terrain.")
+    step = s.currentStep()
+    assert np.all(np.isfinite(s.field(2)))
+    ok, err2 = s.advance(1)
+    assert not ok and err2 == err and s.currentStep() == step
