@@ -107,6 +107,47 @@ def cpu_baseline(pkg, params, steps=10, warmup=2):
                       f"OpenMP rows-parallel as the reference"}
 
 
+ALGO_BYTES_3D = {"f32": 784.0, "f64": 1568.0}  # per node per RK4 step, 3D nine-field (9 fields, 13 coefficients)
+
+
+def measure_3d(pkg, steps, warmup, precision):
+    """400x288x200 DEM-terrain 3D elastic shot; FP64 (precision 0) or FP32 (1)."""
+    from seiswavelab_curvilinear_amd import workloads
+    p, seed = workloads.dem_terrain_400x288x200(steps=warmup + steps + 2)
+    sim = pkg.Simulation3D(pkg.BackendKind.Hip)
+    t = time.perf_counter()
+    ok, err = sim.initialize(p, 0, precision)
+    if not ok:
+        raise RuntimeError(err)
+    label = "FP64" if precision == 0 else "FP32"
+    log(f"[bench] 3D {label}: initialized in {time.perf_counter() - t:.1f} s, dt={sim.dt():.6g} s")
+    ok, err = sim.advance(warmup)
+    if not ok:
+        raise RuntimeError(err)
+    sim.profile(True)
+    t0 = time.perf_counter()
+    ok, err = sim.advance(steps)
+    wall = time.perf_counter() - t0
+    if not ok:
+        raise RuntimeError(err)
+    timing = sim.kernel_timing()
+    d = sim.diagnostics()
+    sim.close()
+    nodes = timing["nodes_per_launch"]
+    avg_ms = timing["stage_ms"] / max(timing["stage_launches"], 1)
+    prec = "f64" if precision == 0 else "f32"
+    per_launch = ALGO_BYTES_3D[prec] / 4 * nodes
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9
+    log(f"[bench] 3D {label}: {steps} steps in {wall:.3f} s ({1e3 * wall / steps:.3f} ms/step)")
+    return {"value": round(nodes * steps / wall / 1e9, 4), "unit": "G pt-updates/s",
+            "ms_per_step": round(1e3 * wall / steps, 4), "dtype": prec, "seed": seed, "finite": d["allFinite"],
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "avg_launch_ms": round(avg_ms, 5),
+                         "algorithmic_bytes_per_launch": per_launch, "nodes_per_launch": nodes,
+                         "metric_planes": timing["metric_planes"], "material_by_depth": timing["material_by_depth"],
+                         "vgprs": timing["vgprs"]}}
+
+
 def traffic_from_profile(prec):
     """HBM bytes per stage launch from the committed rocprofv3 PMC summary, if present."""
     path = ROOT / "profiles" / f"stage_kernel_elastic2d_5061x2031_{prec}_pmc.json"
@@ -130,6 +171,7 @@ def main():
     ap.add_argument("--nz", type=int, default=2001)
     ap.add_argument("--nt", type=int, default=1, help="nontemporal stores in the interior kernel")
     ap.add_argument("--stored-metrics", type=int, default=0, help="read a, b, J from memory instead of recomputing")
+    ap.add_argument("--steps-3d", type=int, default=40, help="timed steps of the 400x288x200 3D shot (0 = skip)")
     ap.add_argument("--probe-bytes", type=int, default=1 << 30,
                     help="size of the plain copy used as the sustained-bandwidth reference (0 = skip)")
     args = ap.parse_args()
@@ -165,6 +207,9 @@ def main():
         extra["fp64"] = {"value": round(r64["nodes"] * args.fp64_steps / r64["wall"] / 1e9, 4),
                          "ms_per_step": round(1e3 * r64["wall"] / args.fp64_steps, 4), "roofline": roof64}
         w.params.precision = pkg.Precision.Float32
+    if args.steps_3d > 0:
+        extra["3d_elastic_400x288x200"] = {"fp64": measure_3d(pkg, args.steps_3d, 5, 0),
+                                           "fp32": measure_3d(pkg, args.steps_3d, 5, 1)}
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(pkg, w.params)

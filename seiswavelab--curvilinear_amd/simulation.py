@@ -112,11 +112,13 @@ class Simulation:
         return self._params
 
     def layout(self) -> dict:
+        self._need()
         o = np.zeros(10, np.int64)
         native.lib().swl_sim2d_layout(self._h, _dp(o))
         return dict(zip(LAYOUT_NAMES, (int(v) for v in o)))
 
     def diagnostics(self) -> dict:
+        self._need()
         o = np.zeros(len(DIAGNOSTIC_NAMES))
         native.lib().swl_sim2d_diagnostics(self._h, _dp(o))
         d = dict(zip(DIAGNOSTIC_NAMES, o.tolist()))
@@ -124,6 +126,7 @@ class Simulation:
         return d
 
     def receiver_arrays(self):
+        self._need()
         L = self.layout()
         nr, n = L["receivers"], L["records"]
         vx, vz = np.zeros((nr, n)), np.zeros((nr, n))
@@ -137,11 +140,13 @@ class Simulation:
                 for r in range(a["vx"].shape[0])]
 
     def probe_indices(self) -> np.ndarray:
+        self._need()
         o = np.zeros(self.layout()["receivers"], np.int64)
         native.lib().swl_sim2d_probe_indices(self._h, _dp(o))
         return o
 
     def model_array(self, name_or_index) -> np.ndarray:
+        self._need()
         which = MODEL_ARRAYS.get(name_or_index, name_or_index)
         L = self.layout()
         n = L["nx"] * L["nz"]
@@ -151,6 +156,7 @@ class Simulation:
         return o
 
     def snapshot_field(self, f: int) -> np.ndarray:
+        self._need()
         L = self.layout()
         o = np.zeros(L["nx"] * L["nz"])
         native.call("swl_sim2d_snapshot", self._h, int(f), _dp(o))
@@ -169,6 +175,7 @@ class Simulation:
         return out
 
     def pml(self) -> np.ndarray:
+        self._need()
         L = self.layout()
         o = np.zeros(max(L["cpml"], 1))
         native.call("swl_sim2d_pml", self._h, _dp(o))
@@ -176,9 +183,11 @@ class Simulation:
 
     def profile(self, enable: bool = True):
         """Per-launch HIP-event timing of the stage and finalize kernels (resets counters)."""
+        self._need()
         native.call("swl_sim2d_profile", self._h, int(enable))
 
     def kernel_timing(self) -> dict:
+        self._need()
         o = np.zeros(9)
         native.call("swl_sim2d_kernel_timing", self._h, _dp(o))
         return {"stage_ms": o[0], "stage_launches": int(o[1]), "finalize_ms": o[2], "finalize_launches": int(o[3]),
@@ -187,9 +196,11 @@ class Simulation:
 
     def set_option(self, name: str, value: int):
         """Engine tuning switch (currently "nontemporal_stores")."""
+        self._need()
         native.call("swl_sim2d_set_option", self._h, name.encode(), int(value))
 
     def readFrame(self, component: int, includeAbsorber: bool = False):
+        self._need()
         L = self.layout()
         p = self._params
         nx = L["nx"] - 6 if includeAbsorber else p.nx
@@ -201,6 +212,7 @@ class Simulation:
         return True, "", out.reshape(nz, nx)
 
     def readProbes(self, nodes):
+        self._need()
         xz = np.ascontiguousarray(np.asarray(nodes, np.int32).reshape(-1, 2))
         out = np.zeros((xz.shape[0], 5))
         rc = native.lib().swl_sim2d_read_probes(self._h, _dp(xz), xz.shape[0], _dp(out))
@@ -304,11 +316,13 @@ class Simulation3D:
         return self._params
 
     def layout(self) -> dict:
+        self._need()
         o = np.zeros(5, np.int64)
         native.lib().swl_sim3d_layout(self._h, _dp(o))
         return dict(zip(["nx", "ny", "nz", "receivers", "records"], (int(v) for v in o)))
 
     def diagnostics(self) -> dict:
+        self._need()
         o = np.zeros(len(DIAGNOSTIC_NAMES_3D))
         native.lib().swl_sim3d_diagnostics(self._h, _dp(o))
         d = dict(zip(DIAGNOSTIC_NAMES_3D, o.tolist()))
@@ -316,6 +330,7 @@ class Simulation3D:
         return d
 
     def receiver_arrays(self):
+        self._need()
         L = self.layout()
         nr, n = L["receivers"], L["records"]
         vx, vy, vz = np.zeros((nr, n)), np.zeros((nr, n)), np.zeros((nr, n))
@@ -329,12 +344,14 @@ class Simulation3D:
                 for r in range(a["vz"].shape[0])]
 
     def field(self, component: int) -> np.ndarray:
+        self._need()
         L = self.layout()
         o = np.zeros(L["nx"] * L["ny"] * L["nz"])
         native.call("swl_sim3d_snapshot", self._h, int(component), _dp(o))
         return o
 
     def model_array(self, name_or_index) -> np.ndarray:
+        self._need()
         which = MODEL_ARRAYS_3D.get(name_or_index, name_or_index)
         L = self.layout()
         n = L["nx"] * L["ny"] * L["nz"]
@@ -343,6 +360,7 @@ class Simulation3D:
         return o
 
     def readDisplayFrame(self, x: int, y: int, z: int, component: int):
+        self._need()
         p = self._params
         out = np.zeros(p.nx * p.nz + p.ny * p.nz + p.nx * p.ny, np.float32)
         rc = native.lib().swl_sim3d_read_display_frame(self._h, int(x), int(y), int(z), int(component), _dp(out))
@@ -352,9 +370,11 @@ class Simulation3D:
         return True, "", [out[:a].reshape(p.nz, p.nx), out[a:a + b].reshape(p.nz, p.ny), out[a + b:].reshape(p.ny, p.nx)]
 
     def profile(self, enable: bool = True):
+        self._need()
         native.call("swl_sim3d_profile", self._h, int(enable))
 
     def kernel_timing(self) -> dict:
+        self._need()
         o = np.zeros(6)
         native.call("swl_sim3d_kernel_timing", self._h, _dp(o))
         return {"stage_ms": o[0], "stage_launches": int(o[1]), "nodes_per_launch": int(o[2]), "vgprs": int(o[3]),

@@ -112,3 +112,35 @@ def computational_nodes(p: Parameters) -> int:
         nx += 2 * wx
         nz += wz
     return nx * nz
+
+
+def dem_terrain_400x288x200(seed: int = 4, nx: int = 400, ny: int = 288, nz: int = 200, steps: int = 1500):
+    """400x288x200 at h=25 m: DEM = seeded GRF (150 m RMS, 800 m correlation) sampled on the model's
+    400x288 grid with datum = mean, flattened at 2000 m; two layers {300 m: 1500/800/2000,
+    basement 3500/2000/2500} by grid index; 8 Hz isotropic moment 5 nodes deep (the shallowest the
+    reference accepts); 20-node sponge; CFL 0.8; receivers on a 32x32 surface array every 4th node
+    (the reference's 1024-trace cap).  Seeds whose relief violates the 2.6x transition rule are
+    skipped (the next seed is used)."""
+    from .params import (LayerDepthMode3D, MaterialLayer3D, ReceiverArrayKind3D, SimulationParameters,
+                         SurfaceGrid3D)
+    h, transition = 25.0, 2000.0
+    while True:
+        rng = np.random.default_rng(seed)
+        z = grf_2d(ny, nx, h, 800.0, 150.0, rng)
+        if 2.6 * np.abs(z).max() < transition:
+            break
+        seed += 1
+    dem = SurfaceGrid3D(nx=nx, ny=ny, spacing=h, originX=0.0, originY=0.0, datum=1000.0, elevations=1000.0 + z)
+    p = SimulationParameters(nx=nx, ny=ny, nz=nz, spacing=h, vp=3500.0, vs=2000.0, density=2500.0,
+                             layeredMaterial=True, layerDepthMode=LayerDepthMode3D.GridIndex,
+                             layers=[MaterialLayer3D("cover", 300, 1500, 800, 2000),
+                                     MaterialLayer3D("basement", 0, 3500, 2000, 2500)],
+                             dominantFrequency=8.0, sourceX=-1, sourceY=-1, sourceZ=5,
+                             source=SourceMechanism(SourceKind.IsotropicMoment), duration=10.0, maxSteps=steps,
+                             cflSafety=0.8, topographyAmplitude=0.0, terrainTransitionDepth=transition,
+                             surfaceGrid=dem)
+    r = p.receiverArray
+    r.kind, r.startX, r.startY, r.z = ReceiverArrayKind3D.GridXY, 0, 0, 0
+    r.countX, r.countY, r.strideX, r.strideY = 32, 32, 4, 4
+    p.absorbingBoundary.cells = (20, 20, 20)
+    return p, seed
