@@ -238,6 +238,44 @@ int64_t swl_pml_slot2d(int nx_pad, int nz_pad, int width_x, int width_z, int64_t
 size_t swl_params2d_size(void);
 size_t swl_params3d_size(void);
 
+/* ---------------- 2D multi-shot batch (no counterpart in the reference) ----------------
+   The reference steps one shot per session (CudaEngine2D.cu:143-152).  A runner steps many 2D engines
+   together: per RK stage one launch covers the tiles of every shot that shares a precision, a GSLS
+   mechanism count and the stencil direction pair; chunks of steps are captured into hipGraphs and
+   replayed; an autotuner picks the persistent grid size.  Each shot keeps the semantics of
+   swl_engine2d_run (device-side fault rule, receiver gather, its own dt / frequency / step number). */
+typedef struct swl_batch_shot {
+    swl_engine2d *engine;
+    int first_step, count;                        /* steps first_step .. first_step + count - 1 */
+    double dt, frequency, scale, peak_energy;     /* as swl_engine2d_run */
+    double *diag, *samples;                       /* host rows, as swl_engine2d_run (count rows) */
+    swl_run2d result;                             /* out */
+} swl_batch_shot;
+typedef struct swl_batch2d_runner swl_batch2d_runner;
+int swl_batch2d_runner_create(int device, swl_batch2d_runner **out);
+void swl_batch2d_runner_destroy(swl_batch2d_runner *r);
+/* options: "graphs" (default 1), "autotune" (1), "grid" (0 = tuned / one block per tile), "chunk" (32 steps) */
+int swl_batch2d_runner_set_option(swl_batch2d_runner *r, const char *name, int value);
+int swl_batch2d_run(swl_batch2d_runner *r, swl_batch_shot *shots, int n);
+/* out[10]: chosen grid (-1 untuned, 0 = one block per tile), graph captures, graph replays, direct chunks,
+   chunks, autotuner candidate ms x4, chunk length */
+int swl_batch2d_runner_stats(const swl_batch2d_runner *r, double *out);
+
+/* Session-level batch over swl_sim2d sessions created on the runner's device (not owned): advance steps
+   every member up to `count` steps (each stops at its own total).  A member's fault latches on that
+   session only (swl_sim2d_fault) and the others continue. */
+typedef struct swl_batch2d swl_batch2d;
+int swl_batch2d_create(int device, swl_batch2d **out);
+void swl_batch2d_destroy(swl_batch2d *b);
+int swl_batch2d_add(swl_batch2d *b, swl_sim2d *session);
+int swl_batch2d_remove(swl_batch2d *b, swl_sim2d *session);
+int swl_batch2d_size(const swl_batch2d *b);
+int swl_batch2d_advance(swl_batch2d *b, int count);
+int swl_batch2d_set_option(swl_batch2d *b, const char *name, int value);
+int swl_batch2d_stats(const swl_batch2d *b, double *out);
+/* "" when the session has no latched fault */
+const char *swl_sim2d_fault(const swl_sim2d *s);
+
 /* ---------------- 3D engine (replaces CudaEngine3D, src/three_d/CudaEngine3D.hpp:7-24) ---------------- */
 /* FP64 host mesh in padded storage q = i + nx*(j + ny*k) (Point::Mesh, src/three_d/PointKernels3D.hpp:13-21).
    Material arrays are per node (the homogeneous case repeats the constants).  The optional

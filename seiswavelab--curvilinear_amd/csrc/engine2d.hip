@@ -24,9 +24,12 @@
 #include "common.hpp"
 #include "swl.h"
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <sstream>
+#include <string>
 #include <vector>
 
 namespace swl {
@@ -704,7 +707,44 @@ template <class R> struct FinalArgs {
     int nrec, row, step;
     double dt, frequency;
 };
-template <class R> __global__ __launch_bounds__(THREADS) void finalize_kernel(FinalArgs<R> A) {
+template <class R> __global__ __launch_bounds__(THREADS) void finalize_kernel(FinalArgs<R> A);
+template <class R> __device__ void finalize_body(const FinalArgs<R> &A);
+template <class R> __global__ __launch_bounds__(THREADS) void finalize_kernel(FinalArgs<R> A) { finalize_body(A); }
+
+// ---- multi-shot batch: one launch per RK stage walks the tiles of every shot in the batch.
+// work[item] = (shot, tile): tile >= 0 a border tile index, tile | BATCH_INTERIOR an interior tile
+// (already XCD-band mapped), -1 padding.  args[shot] are the shot's stage arguments for this launch;
+// kind < 0 marks a shot that has no work in it (finished, or another direction pair).  Grid-stride
+// loop, so the grid may be smaller than the work list (persistent blocks).
+constexpr int BATCH_INTERIOR = 1 << 30;
+template <class R, int DX, int DZ, int MECH>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<R, MECH>))) void batch_stage_kernel(
+    const StageArgs<R> *__restrict__ args, const int2 *__restrict__ work, int nwork) {
+    __shared__ Partial sh[THREADS];
+    for (int item = blockIdx.x; item < nwork; item += gridDim.x) {
+        const int2 w = work[item];
+        if (w.y < 0) continue;
+        const StageArgs<R> &A = args[w.x];
+        if (A.kind < 0 || *A.status) continue;
+        if (w.y & BATCH_INTERIOR) interior_body<R, DX, DZ, MECH>(A, w.y & ~BATCH_INTERIOR, sh);
+        else general_body<R, DX, DZ, MECH>(A, w.y, sh);
+        __syncthreads(); // the next item reuses sh
+    }
+}
+// one block per shot: all partials of the shot's last stage -> fault rule, diagnostics row, receivers
+template <class R> __global__ __launch_bounds__(THREADS) void batch_final_kernel(const FinalArgs<R> *__restrict__ fa) {
+    const FinalArgs<R> &A = fa[blockIdx.x];
+    if (A.n <= 0) return;
+    finalize_body(A);
+}
+// zeroes the per-shot run counters at the start of a captured chunk (no memset nodes in graphs)
+__global__ void batch_begin_kernel(int *const *status, int *const *accepted, double *const *peak, const double *peak0,
+                                   int n) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) *status[s] = 0, *accepted[s] = 0, *peak[s] = peak0[s];
+}
+
+template <class R> __device__ void finalize_body(const FinalArgs<R> &A) {
     __shared__ Partial sh[THREADS];
     __shared__ int verdict;
     if (*A.status) return;
@@ -797,3 +837,4 @@ __global__ __launch_bounds__(256) void stream_copy_kernel(const float4 *__restri
 } // namespace swl
 
 #include "engine2d_host.inl"
+#include "batch2d_host.inl"

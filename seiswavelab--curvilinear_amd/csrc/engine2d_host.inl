@@ -5,8 +5,14 @@
 namespace swl {
 namespace {
 
+inline uint64_t next_engine_uid() {
+    static std::atomic<uint64_t> counter{0};
+    return ++counter;
+}
+
 template <class R> struct Engine2D {
     static constexpr int NV = Lane<R>::N;
+    const uint64_t uid = next_engine_uid(); // identity for caches keyed on engines (addresses can be reused)
     int device = 0, nrec = 0, fields = 5, mech = 0, precision = 0;
     int nx = 0, nz = 0, ld = 0, off = ORIGIN;
     size_t count = 0; // field stride (elements)
@@ -257,9 +263,7 @@ template <class R> struct Engine2D {
 
     // slot = RK stage 0..3; with profiling on, one event pair brackets the four back-to-back stage launches
     // (an event between launches would itself add a dispatch gap)
-    template <int DX, int DZ> void launch_both(StageArgs<R> a, int slot) {
-        a.tiles = generalTiles.as<int>(), a.ngeneral = nGeneral, a.gblocks = (nGeneral + 7) / 8 * 8, a.gntx = ntx;
-        a.ninterior = nInterior, a.rtx = rtx, a.nt = nt;
+    template <int DX, int DZ> void launch_both(const StageArgs<R> &a, int slot) {
         const int blocks = a.gblocks + nInterior;
         if (slot == 0) mark(0, true);
         switch (mech) {
@@ -288,27 +292,33 @@ template <class R> struct Engine2D {
         return (1 - 2 * a * a) * std::exp(-a * a);
     }
 
-    // Enqueue one RK4 step from buffer c; returns the buffer holding the new state.
-    int enqueue_step(int c, double dt, double frequency, double scale, int step) {
+    // Arguments of RK stage s (0..3) of step `step` taken from buffer c (three rotating buffers:
+    // stages read c, b1, b2, b1 and write b1, b2, b1, b2; the step ends in b2 = (c + 2) % 3).
+    StageArgs<R> stage_args(int c, double dt, double frequency, double scale, int step, int s) const {
         const int b1 = (c + 1) % 3, b2 = (c + 2) % 3;
         const int in[4] = {c, b1, b2, b1}, out[4] = {b1, b2, b1, b2};
         const double times[4] = {step * dt, (step + .5) * dt, (step + .5) * dt, (step + 1) * dt};
         const R weights[4] = {(R)(dt / 2), (R)(dt / 2), (R)dt, (R)dt};
-        for (int s = 0; s < 4; ++s) {
-            StageArgs<R> a{};
-            a.m = m;
-            a.in = S(in[s]), a.base = S(c), a.out = S(out[s]), a.acc = accbuf.as<R>();
-            if (cpml) a.pin = P(in[s]), a.pbase = P(c), a.pout = P(out[s]), a.pacc = paccbuf.as<R>();
-            a.count = count;
-            a.forcing = (R)(scale * pulse(times[s], frequency));
-            a.w = weights[s];
-            a.wy = (R)(s == 0 || s == 3 ? dt / 6 : dt / 3);
-            a.kind = s == 0 ? 0 : s == 3 ? 2 : 1;
-            a.status = status.as<int>();
-            a.partials = partials.as<Partial>();
-            launch_stage(direction(step, s, 0), direction(step, s, 2), a, s);
-        }
-        return b2;
+        StageArgs<R> a{};
+        a.m = m;
+        a.in = S(in[s]), a.base = S(c), a.out = S(out[s]), a.acc = accbuf.as<R>();
+        if (cpml) a.pin = P(in[s]), a.pbase = P(c), a.pout = P(out[s]), a.pacc = paccbuf.as<R>();
+        a.count = count;
+        a.forcing = (R)(scale * pulse(times[s], frequency));
+        a.w = weights[s];
+        a.wy = (R)(s == 0 || s == 3 ? dt / 6 : dt / 3);
+        a.kind = s == 0 ? 0 : s == 3 ? 2 : 1;
+        a.status = status.as<int>();
+        a.partials = partials.as<Partial>();
+        a.tiles = generalTiles.as<int>(), a.ngeneral = nGeneral, a.gblocks = (nGeneral + 7) / 8 * 8, a.gntx = ntx;
+        a.ninterior = nInterior, a.rtx = rtx, a.nt = nt;
+        return a;
+    }
+    // Enqueue one RK4 step from buffer c; returns the buffer holding the new state.
+    int enqueue_step(int c, double dt, double frequency, double scale, int step) {
+        for (int s = 0; s < 4; ++s)
+            launch_stage(direction(step, s, 0), direction(step, s, 2), stage_args(c, dt, frequency, scale, step, s), s);
+        return (c + 2) % 3;
     }
 
     void step(double, double dt, double frequency, double scale, int number) {

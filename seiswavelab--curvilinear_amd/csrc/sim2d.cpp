@@ -3,9 +3,7 @@
 // initialize() is transactional (a failed build leaves no session), advance() keeps the
 // reference's per-step semantics (inspection, fault rule with rollback and latched fault,
 // receiver lerp resampling) while the engine runs chunks of steps on the device.
-#include "model2d.hpp"
-#include "common.hpp"
-#include "swl.h"
+#include "session2d.hpp"
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -21,80 +19,6 @@ std::string &last_error() {
     return message;
 }
 } // namespace swl
-
-struct swl_sim2d {
-    swl::Model2D model;
-    swl_engine2d *engine = nullptr;
-    int device = 0;
-    int step = 0;
-    std::string fault;
-    double energy = 0, peak = 0, maxVelocity = 0, traction = 0;
-    bool finite = true;
-    double initializationMs = 0, totalStepMs = 0, lastStepMs = 0;
-    std::vector<std::vector<double>> vx, vz; // per receiver, recorded samples
-    std::vector<std::array<double, 2>> previous;
-    std::vector<double> diagRows, sampleRows;
-    ~swl_sim2d() {
-        if (engine) swl_engine2d_destroy(engine);
-    }
-    double recordDt() const {
-        return model.p.sample_interval_ms > 0 ? model.p.sample_interval_ms * .001 : model.dt;
-    }
-    void clearRecords() {
-        const size_t n = model.receiverX.size();
-        vx.assign(n, {}), vz.assign(n, {});
-        for (size_t r = 0; r < n; ++r) {
-            vx[r].reserve(model.recordReserve), vz[r].reserve(model.recordReserve);
-            vx[r].push_back(0), vz[r].push_back(0);
-        }
-        previous.assign(n, {0, 0});
-    }
-    // CurvilinearSolver2D.cpp:120-147, with the raw probe values of the step just accepted.
-    void sample(const double *raw) {
-        const double interval = recordDt(), dt = model.dt, end = (step + 1) * dt;
-        for (size_t n = vx.front().size(); n * interval <= end + 1e-9 * dt; ++n) {
-            const double a = std::clamp((n * interval - step * dt) / dt, 0.0, 1.0);
-            for (size_t r = 0; r < vx.size(); ++r) {
-                vx[r].push_back(swl::lerpExact(previous[r][0], raw[2 * r], a));
-                vz[r].push_back(swl::lerpExact(previous[r][1], raw[2 * r + 1], a));
-            }
-        }
-        for (size_t r = 0; r < vx.size(); ++r) previous[r] = {raw[2 * r], raw[2 * r + 1]};
-    }
-    void advance(int count) {
-        if (!fault.empty()) throw std::runtime_error(fault);
-        constexpr int chunk = 512;
-        const size_t nrec = model.receiverX.size();
-        const double scale = model.amplitude / (model.p.spacing * model.p.spacing);
-        while (count > 0 && step < model.total) {
-            const int n = std::min({count, model.total - step, chunk});
-            diagRows.resize(4 * static_cast<size_t>(n));
-            sampleRows.resize(2 * nrec * static_cast<size_t>(n));
-            swl_run2d res{};
-            const auto start = std::chrono::steady_clock::now();
-            if (swl_engine2d_run(engine, step, n, model.dt, model.p.frequency, scale, peak, diagRows.data(),
-                                 sampleRows.data(), &res)) {
-                fault = "Compute backend failed: " + swl::last_error() + ". Rebuild the session.";
-                throw std::runtime_error(fault);
-            }
-            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count();
-            for (int j = 0; j < res.accepted; ++j) {
-                const double *d = &diagRows[4 * static_cast<size_t>(j)];
-                energy = d[0], maxVelocity = d[1], traction = d[2], finite = d[3] != 0;
-                sample(&sampleRows[2 * nrec * static_cast<size_t>(j)]);
-                ++step;
-                peak = std::max(peak, energy);
-            }
-            totalStepMs += ms;
-            lastStepMs = res.accepted ? ms / res.accepted : ms;
-            if (res.faulted) {
-                fault = "2D step rejected: nonfinite fields or excessive late energy growth. Rebuild the session.";
-                throw std::runtime_error(fault);
-            }
-            count -= n;
-        }
-    }
-};
 
 namespace {
 int64_t layoutValue(const swl_sim2d &s, int i) {

@@ -76,6 +76,20 @@ SIGNATURES = {
     "swl_pml_slot2d": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int]),
     "swl_params2d_size": (C.c_size_t, []),
     "swl_params3d_size": (C.c_size_t, []),
+    "swl_batch2d_runner_create": (C.c_int, [C.c_int, P(vp)]),
+    "swl_batch2d_runner_destroy": (None, [vp]),
+    "swl_batch2d_runner_set_option": (C.c_int, [vp, C.c_char_p, C.c_int]),
+    "swl_batch2d_run": (C.c_int, [vp, vp, C.c_int]),
+    "swl_batch2d_runner_stats": (C.c_int, [vp, vp]),
+    "swl_batch2d_create": (C.c_int, [C.c_int, P(vp)]),
+    "swl_batch2d_destroy": (None, [vp]),
+    "swl_batch2d_add": (C.c_int, [vp, vp]),
+    "swl_batch2d_remove": (C.c_int, [vp, vp]),
+    "swl_batch2d_size": (C.c_int, [vp]),
+    "swl_batch2d_advance": (C.c_int, [vp, C.c_int]),
+    "swl_batch2d_set_option": (C.c_int, [vp, C.c_char_p, C.c_int]),
+    "swl_batch2d_stats": (C.c_int, [vp, vp]),
+    "swl_sim2d_fault": (C.c_char_p, [vp]),
     "swl_engine3d_create": (C.c_int, [vp, vp, C.c_int, C.c_int, P(vp)]),
     "swl_engine3d_destroy": (None, [vp]),
     "swl_engine3d_step": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]),

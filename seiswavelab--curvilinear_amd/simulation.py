@@ -194,6 +194,11 @@ class Simulation:
                 "interior_tiles": int(o[4]), "vgprs": int(o[5]), "nodes_per_launch": int(o[6]),
                 "border_tiles": int(o[7]), "metric_on_the_fly": bool(o[8])}
 
+    def fault(self) -> str:
+        """The latched fault message ("" if none)."""
+        self._need()
+        return native.lib().swl_sim2d_fault(self._h).decode()
+
     def set_option(self, name: str, value: int):
         """Engine tuning switch (currently "nontemporal_stores")."""
         self._need()
@@ -379,3 +384,55 @@ class Simulation3D:
         native.call("swl_sim3d_kernel_timing", self._h, _dp(o))
         return {"stage_ms": o[0], "stage_launches": int(o[1]), "nodes_per_launch": int(o[2]), "vgprs": int(o[3]),
                 "metric_planes": int(o[4]), "material_by_depth": bool(o[5])}
+
+
+class Batch2D:
+    """Steps many 2D sessions together (one launch per RK stage for all of them; graph-captured chunks).
+
+    Sessions keep their own semantics: ``advance`` moves every member up to ``count`` steps, each
+    stops at its own total, and a member's fault latches on that member only (``Simulation.fault``)."""
+
+    STAT_NAMES = ["grid", "captures", "replays", "direct_chunks", "chunks", "tune_ms_0", "tune_ms_1", "tune_ms_2",
+                  "tune_ms_3", "chunk"]
+
+    def __init__(self, device: int = 0):
+        out = C.c_void_p()
+        native.call("swl_batch2d_create", int(device), C.byref(out))
+        self._h = out
+        self._members = []
+
+    def close(self):
+        if self._h:
+            native.lib().swl_batch2d_destroy(self._h)
+            self._h = None
+            self._members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, sim: "Simulation"):
+        sim._need()
+        native.call("swl_batch2d_add", self._h, sim._h)
+        self._members.append(sim)  # keeps the session alive while it is a member
+
+    def remove(self, sim: "Simulation"):
+        native.call("swl_batch2d_remove", self._h, sim._h)
+        self._members.remove(sim)
+
+    def __len__(self):
+        return native.lib().swl_batch2d_size(self._h)
+
+    def advance(self, count: int):
+        rc = native.lib().swl_batch2d_advance(self._h, int(count))
+        return (rc == 0), ("" if rc == 0 else native.last_error())
+
+    def set_option(self, name: str, value: int):
+        native.call("swl_batch2d_set_option", self._h, name.encode(), int(value))
+
+    def stats(self) -> dict:
+        o = np.zeros(len(self.STAT_NAMES))
+        native.lib().swl_batch2d_stats(self._h, _dp(o))
+        return dict(zip(self.STAT_NAMES, o.tolist()))
