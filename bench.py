@@ -148,6 +148,75 @@ def measure_3d(pkg, steps, warmup, precision):
                          "vgprs": timing["vgprs"]}}
 
 
+def measure_batch(pkg, shots, steps, full_steps=3000):
+    """64-shot GSLS-3 viscoelastic batch (FP32 state, FP64 metrics / closure) through one Batch2D versus the
+    plain single-shot FP64 path (each shot alone, one after another).  Both run the first `steps` steps of
+    every shot; shots/s is quoted for the full `full_steps`-step shots from the measured time per step."""
+    import numpy as np
+    from seiswavelab_curvilinear_amd import workloads
+    p32 = workloads.gsls_batch(shots=shots, precision=pkg.Precision.Float32, steps=full_steps)
+    p64 = workloads.gsls_batch(shots=shots, precision=pkg.Precision.Float64, steps=full_steps)
+    t = time.perf_counter()
+    sims = []
+    for p in p32:
+        s = pkg.Simulation()
+        ok, err = s.initialize(p, pkg.BackendKind.Hip, 0)
+        if not ok:
+            raise RuntimeError(err)
+        sims.append(s)
+    nodes = sum((s.layout()["nx"] - 6) * (s.layout()["nz"] - 6) for s in sims)
+    log(f"[bench] batch: {shots} FP32 GSLS-3 sessions built in {time.perf_counter() - t:.1f} s, "
+        f"{nodes / shots / 1e6:.3f} M computational nodes per shot on average")
+    batch = pkg.Batch2D(0)
+    for s in sims:
+        batch.add(s)
+    warm = min(16, steps)
+    ok, err = batch.advance(warm)  # first chunks: graph capture and the autotuner's timed candidates
+    if not ok:
+        raise RuntimeError(err)
+    t0 = time.perf_counter()
+    ok, err = batch.advance(steps - warm)
+    wall = time.perf_counter() - t0
+    if not ok:
+        raise RuntimeError(err)
+    stats = batch.stats()
+    per_step_batch = wall / (steps - warm)
+    log(f"[bench] batch: {steps - warm} steps of all {shots} shots in {wall:.3f} s; stats {stats}")
+    # plain single-shot FP64 path on the same shots
+    plain_wall, errs = 0.0, []
+    for i, p in enumerate(p64):
+        s = pkg.Simulation()
+        ok, err = s.initialize(p, pkg.BackendKind.Hip, 0)
+        if not ok:
+            raise RuntimeError(err)
+        ok, err = s.advance(warm)
+        if not ok:
+            raise RuntimeError(err)
+        t0 = time.perf_counter()
+        ok, err = s.advance(steps - warm)
+        plain_wall += time.perf_counter() - t0
+        if not ok:
+            raise RuntimeError(err)
+        a, b = s.receiver_arrays(), sims[i].receiver_arrays()
+        ref = np.concatenate([a["vx"].ravel(), a["vz"].ravel()])
+        val = np.concatenate([b["vx"].ravel(), b["vz"].ravel()])
+        errs.append(float(np.sqrt(np.sum((ref - val) ** 2) / max(np.sum(ref * ref), 1e-300))))
+        s.close()
+    per_step_plain = plain_wall / (steps - warm)
+    batch_sps = shots / (per_step_batch * full_steps)
+    plain_sps = shots / (per_step_plain * full_steps)
+    log(f"[bench] plain FP64: {steps - warm} steps of each shot alone, {plain_wall:.3f} s in total")
+    return {"shots": shots, "steps_timed_per_shot": steps - warm, "full_steps": full_steps,
+            "mean_nodes_per_shot": round(nodes / shots), "batched_shots_per_s": round(batch_sps, 4),
+            "plain_fp64_shots_per_s": round(plain_sps, 4), "speedup": round(batch_sps / plain_sps, 3),
+            "batched_ms_per_step_all_shots": round(1e3 * per_step_batch, 4),
+            "plain_fp64_ms_per_step_sum": round(1e3 * per_step_plain, 4),
+            "batched_G_pt_updates_per_s": round(nodes / per_step_batch / 1e9, 4),
+            "max_gather_rel_l2_fp32_batch_vs_fp64_plain": max(errs), "graph": stats,
+            "note": "shots/s for full 3000-step shots = shots / (measured time per step x 3000); both paths timed "
+                    "on the same steps after the same warm-up"}
+
+
 def traffic_from_profile(prec):
     """HBM bytes per stage launch from the committed rocprofv3 PMC summary, if present."""
     path = ROOT / "profiles" / f"stage_kernel_elastic2d_5061x2031_{prec}_pmc.json"
@@ -172,6 +241,8 @@ def main():
     ap.add_argument("--nt", type=int, default=1, help="nontemporal stores in the interior kernel")
     ap.add_argument("--stored-metrics", type=int, default=0, help="read a, b, J from memory instead of recomputing")
     ap.add_argument("--steps-3d", type=int, default=40, help="timed steps of the 400x288x200 3D shot (0 = skip)")
+    ap.add_argument("--batch-steps", type=int, default=200, help="steps of each of the 64 GSLS shots (0 = skip)")
+    ap.add_argument("--batch-shots", type=int, default=64)
     ap.add_argument("--probe-bytes", type=int, default=1 << 30,
                     help="size of the plain copy used as the sustained-bandwidth reference (0 = skip)")
     args = ap.parse_args()
@@ -210,6 +281,8 @@ def main():
     if args.steps_3d > 0:
         extra["3d_elastic_400x288x200"] = {"fp64": measure_3d(pkg, args.steps_3d, 5, 0),
                                            "fp32": measure_3d(pkg, args.steps_3d, 5, 1)}
+    if args.batch_steps > 16:
+        extra["batch_gsls3"] = measure_batch(pkg, args.batch_shots, args.batch_steps)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(pkg, w.params)

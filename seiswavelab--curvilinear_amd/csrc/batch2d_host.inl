@@ -125,7 +125,9 @@ struct Runner2D {
         const StageArgs<R> *a = B.args.template as<StageArgs<R>>() + L.offset;
         const int2 *w = g.workBuf.template as<int2>();
         const int n = (int)g.work.size();
-#define SWL_BATCH_LAUNCH(DX, DZ, MECH) batch_stage_kernel<R, DX, DZ, MECH><<<L.grid, THREADS, 0, stream>>>(a, w, n)
+#define SWL_BATCH_LAUNCH(DX, DZ, MECH)                                                                               \
+    if (L.grid >= n) batch_stage_kernel<R, DX, DZ, MECH, 0><<<n, THREADS, 0, stream>>>(a, w, n);                     \
+    else batch_stage_kernel<R, DX, DZ, MECH, 1><<<L.grid, THREADS, 0, stream>>>(a, w, n)
 #define SWL_BATCH_MECH(DX, DZ)                                                                                       \
     switch (L.mech) {                                                                                                \
     case 0: SWL_BATCH_LAUNCH(DX, DZ, 0); break;                                                                      \

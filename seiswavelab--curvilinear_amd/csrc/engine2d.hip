@@ -61,6 +61,8 @@ template <class R> struct DMesh {
     const double *tz_, *tslope, *fade, *fadeder; // separable metric profiles (by i / by k)
     int ri0, ri1, rk0, rk1;                      // interior rectangle [ri0,ri1) x [rk0,rk1)
     int si0, si1, sk0, sk1;                      // source support box (inclusive), si1 < 0 if none
+    int by_row;                                  // lambda, mu, rho, dP, dS depend on k only (per-row tables)
+    const R *lam_k, *mu_k, *rho_k, *dP_k[3], *dS_k[3];
 };
 template <class R> __device__ __host__ inline ptrdiff_t at(const DMesh<R> &m, int i, int k) {
     return (ptrdiff_t)m.off + i + (ptrdiff_t)m.ld * k;
@@ -532,7 +534,9 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
 #pragma unroll
             for (int j = 0; j < NV; ++j) a[j] = va[j], b[j] = vb[j], jac[j] = vj[j];
         }
-        const V vrho = vload(m.rho + q), vlam = vload(m.lam + q), vmu = vload(m.mu + q);
+        // layered models: materials depend on the row k only (grid-conformal layers), read per row
+        const V vrho = m.by_row ? V(m.rho_k[k]) : vload(m.rho + q), vlam = m.by_row ? V(m.lam_k[k]) : vload(m.lam + q),
+                vmu = m.by_row ? V(m.mu_k[k]) : vload(m.mu + q);
         R kv[F][NV];
         const bool src = m.si1 >= 0 && k >= m.sk0 && k <= m.sk1 && i0 + NV - 1 >= m.si0 && i0 <= m.si1;
 #pragma unroll
@@ -589,7 +593,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
 #pragma unroll
             for (int l = 0; l < MECH; ++l) {
                 const int kk = 6 + 4 * l;
-                const V vdp = vload(m.dP[l] + q), vds = vload(m.dS[l] + q);
+                const V vdp = m.by_row ? V(m.dP_k[l][k]) : vload(m.dP[l] + q), vds = m.by_row ? V(m.dS_k[l][k]) : vload(m.dS[l] + q);
                 V mem[4];
 #pragma unroll
                 for (int f = 0; f < 4; ++f) mem[f] = vload(A.in + (size_t)(kk + f) * cnt + q);
@@ -718,17 +722,25 @@ template <class R> __global__ __launch_bounds__(THREADS) void finalize_kernel(Fi
 // loop, so the grid may be smaller than the work list (persistent blocks).
 constexpr int BATCH_INTERIOR = 1 << 30;
 template <class R, int DX, int DZ, int MECH>
+__device__ __forceinline__ void batch_item(const StageArgs<R> *__restrict__ args, const int2 w, Partial *sh) {
+    if (w.y < 0) return;
+    const StageArgs<R> &A = args[w.x];
+    if (A.kind < 0 || *A.status) return;
+    if (w.y & BATCH_INTERIOR) interior_body<R, DX, DZ, MECH>(A, w.y & ~BATCH_INTERIOR, sh);
+    else general_body<R, DX, DZ, MECH>(A, w.y, sh);
+}
+// PERSIST = 0: one block per work item (grid = nwork); 1: grid-stride loop over the items
+template <class R, int DX, int DZ, int MECH, int PERSIST>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<R, MECH>))) void batch_stage_kernel(
     const StageArgs<R> *__restrict__ args, const int2 *__restrict__ work, int nwork) {
     __shared__ Partial sh[THREADS];
-    for (int item = blockIdx.x; item < nwork; item += gridDim.x) {
-        const int2 w = work[item];
-        if (w.y < 0) continue;
-        const StageArgs<R> &A = args[w.x];
-        if (A.kind < 0 || *A.status) continue;
-        if (w.y & BATCH_INTERIOR) interior_body<R, DX, DZ, MECH>(A, w.y & ~BATCH_INTERIOR, sh);
-        else general_body<R, DX, DZ, MECH>(A, w.y, sh);
-        __syncthreads(); // the next item reuses sh
+    if constexpr (!PERSIST) {
+        batch_item<R, DX, DZ, MECH>(args, work[blockIdx.x], sh);
+    } else {
+        for (int item = blockIdx.x; item < nwork; item += gridDim.x) {
+            batch_item<R, DX, DZ, MECH>(args, work[item], sh);
+            __syncthreads(); // the next item reuses sh
+        }
     }
 }
 // one block per shot: all partials of the shot's last stage -> fault rule, diagnostics row, receivers

@@ -139,6 +139,7 @@ template <class R> struct Engine2D {
         m.slam = put_plain<double>(21, sl.data(), sl.size()), m.smu = put_plain<double>(22, sm.data(), sm.size());
         m.snrm = put_plain<double>(23, sn.data(), sn.size());
         setup_metric(d);
+        setup_rows(d);
         setup_regions(d);
         for (auto &b : state) b.allocate((size_t)fields * count * sizeof(R));
         accbuf.allocate((size_t)fields * count * sizeof(R));
@@ -191,6 +192,27 @@ template <class R> struct Engine2D {
         m.fade = put_plain<double>(26, d.fade, nz);
         m.fadeder = put_plain<double>(27, d.fade_derivative, nz);
         m.metric = metricOk = 1;
+    }
+
+    // Per-row material tables when every material array is constant along each row (layered models,
+    // grid-conformal by k); the interior then reads one value per row instead of streaming the arrays.
+    DeviceBuffer rowTables;
+    void setup_rows(const swl_mesh2d_desc &d) {
+        m.by_row = 0;
+        std::vector<const double *> arrays = {d.lambda, d.mu, d.rho};
+        for (int l = 0; l < mech; ++l) arrays.push_back(d.delta_p[l]), arrays.push_back(d.delta_s[l]);
+        for (const double *a : arrays)
+            for (int k = 0; k < nz; ++k)
+                for (int i = 1; i < nx; ++i)
+                    if (std::memcmp(&a[(size_t)i + (size_t)nx * k], &a[(size_t)nx * k], 8)) return;
+        std::vector<R> table(arrays.size() * (size_t)nz);
+        for (size_t a = 0; a < arrays.size(); ++a)
+            for (int k = 0; k < nz; ++k) table[a * nz + k] = (R)arrays[a][(size_t)nx * k];
+        rowTables.upload(table.data(), table.size() * sizeof(R));
+        const R *base = rowTables.as<R>();
+        m.lam_k = base, m.mu_k = base + nz, m.rho_k = base + 2 * nz;
+        for (int l = 0; l < mech; ++l) m.dP_k[l] = base + (3 + 2 * l) * (size_t)nz, m.dS_k[l] = base + (4 + 2 * l) * (size_t)nz;
+        m.by_row = 1;
     }
 
     // Interior rectangle: depth >= 4, outside every CFS-PML slot, zero sponge damping; x-extent

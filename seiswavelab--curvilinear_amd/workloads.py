@@ -144,3 +144,32 @@ def dem_terrain_400x288x200(seed: int = 4, nx: int = 400, ny: int = 288, nz: int
     r.countX, r.countY, r.strideX, r.strideY = 32, 32, 4, 4
     p.absorbingBoundary.cells = (20, 20, 20)
     return p, seed
+
+
+def gsls_shot(rng: np.random.Generator, precision: Precision = Precision.Float32, steps: int = 3000) -> Parameters:
+    """One shot of the 64-shot viscoelastic batch: nx ~ logU[300, 2000], nz ~ logU[150, 800] (independent),
+    h in {1, 2, 4} m; Qp ~ U[20, 80], Qs = Qp/2, three-mechanism GSLS fitted over [0.2 f0, 3 f0] with the
+    reference frequency f0; Ricker f0 ~ U[15, 45] Hz; vertical force at depth ~ U[0, 10] m (snapped to a
+    node); receivers along the surface every U[2, 10] m (snapped, at most 1024); 20-node CFS-PML; the three
+    layers of THREE_LAYERS; cosine terrain with relief 2 % of the depth and wavelength 30 % of the width."""
+    nx = int(round(math.exp(rng.uniform(math.log(300), math.log(2000)))))
+    nz = int(round(math.exp(rng.uniform(math.log(150), math.log(800)))))
+    h = float(rng.choice([1.0, 2.0, 4.0]))
+    qp = float(rng.uniform(20, 80))
+    f0 = float(rng.uniform(15, 45))
+    depth = float(rng.uniform(0, 10))
+    spacing = float(rng.uniform(2, 10))
+    stride = max(1, int(round(spacing / h)))
+    count = min(1024, (nx - 1) // stride + 1)
+    return Parameters(precision=precision, nx=nx, nz=nz, spacing=h, frequency=f0, duration=10.0, cfl=0.5,
+                      maxSteps=steps, sourceX=-1, sourceZ=int(round(depth / h)),
+                      source=SourceMechanism(SourceKind.DirectionalForce), absorber=Absorber.CfsPml, spongeX=20,
+                      spongeZ=20, receiverStart=0, receiverCount=count, receiverStride=stride, receiverDepth=0,
+                      layered=True, layers=list(THREE_LAYERS), relief=0.02 * nz * h, wavelength=0.3 * nx * h,
+                      attenuation=GslsSettings(enabled=True, mechanisms=3, minFrequency=0.2 * f0,
+                                               maxFrequency=3 * f0, referenceFrequency=f0, qp=qp, qs=qp / 2))
+
+
+def gsls_batch(seed: int = 3, shots: int = 64, precision: Precision = Precision.Float32, steps: int = 3000):
+    rng = np.random.default_rng(seed)
+    return [gsls_shot(rng, precision, steps) for _ in range(shots)]
