@@ -217,6 +217,36 @@ def measure_batch(pkg, shots, steps, full_steps=3000):
                     "on the same steps after the same warm-up"}
 
 
+def measure_serving(pkg, count, steps, rate=20.0):
+    """Poisson stream (20 req/s) of 2D shots (elastic / GSLS-3, FP32) through the continuous-batching server
+    versus the plain path (one shot at a time, FP64, arrival order)."""
+    import copy
+    import numpy as np
+    from seiswavelab_curvilinear_amd import serving, workloads
+    reqs = list(workloads.serving_requests(count=count, rate=rate, steps=steps))
+    window = reqs[-1][1]
+    served = serving.Server(chunk=32).run(copy.deepcopy(reqs))
+    log(f"[bench] serving: {count} requests, makespan {served.wall:.2f} s; {served.summary()}")
+    plain = serving.serve_sequential(copy.deepcopy(reqs), precision=pkg.Precision.Float64)
+    log(f"[bench] plain FP64 sequential: makespan {plain.wall:.2f} s; {plain.summary()}")
+    errs = []
+    for a, b in zip(served.results, plain.results):
+        ga = np.concatenate([a.traces["vx"].ravel(), a.traces["vz"].ravel()])
+        gb = np.concatenate([b.traces["vx"].ravel(), b.traces["vz"].ravel()])
+        errs.append(float(np.sqrt(np.sum((ga - gb) ** 2) / max(np.sum(gb * gb), 1e-300))))
+    node_steps = sum(r.nodes * r.steps for r in served.results)
+    s, q = served.summary(), plain.summary()
+    return {"requests": count, "rate_per_s": rate, "steps_per_request": steps,
+            "mean_nodes": round(np.mean([r.nodes for r in served.results])),
+            "offered_G_pt_updates_per_s": round(node_steps / window / 1e9, 4),
+            "served_G_pt_updates_per_s": round(node_steps / served.wall / 1e9, 4),
+            "served": s, "plain_fp64_sequential": q,
+            "shots_per_s_ratio": round(s["shots_per_s"] / q["shots_per_s"], 3),
+            "p99_ratio_plain_over_served": round(q["p99_s"] / s["p99_s"], 3),
+            "max_gather_rel_l2_served_fp32_vs_plain_fp64": max(errs), "graph": served.stats,
+            "note": "each request's latency = completion - arrival; both paths see the same arrival times"}
+
+
 def traffic_from_profile(prec):
     """HBM bytes per stage launch from the committed rocprofv3 PMC summary, if present."""
     path = ROOT / "profiles" / f"stage_kernel_elastic2d_5061x2031_{prec}_pmc.json"
@@ -243,6 +273,8 @@ def main():
     ap.add_argument("--steps-3d", type=int, default=40, help="timed steps of the 400x288x200 3D shot (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=200, help="steps of each of the 64 GSLS shots (0 = skip)")
     ap.add_argument("--batch-shots", type=int, default=64)
+    ap.add_argument("--serve-requests", type=int, default=48, help="requests of the serving stream (0 = skip)")
+    ap.add_argument("--serve-steps", type=int, default=500, help="steps per served request")
     ap.add_argument("--probe-bytes", type=int, default=1 << 30,
                     help="size of the plain copy used as the sustained-bandwidth reference (0 = skip)")
     args = ap.parse_args()
@@ -283,6 +315,8 @@ def main():
                                            "fp32": measure_3d(pkg, args.steps_3d, 5, 1)}
     if args.batch_steps > 16:
         extra["batch_gsls3"] = measure_batch(pkg, args.batch_shots, args.batch_steps)
+    if args.serve_requests > 0:
+        extra["serving"] = measure_serving(pkg, args.serve_requests, args.serve_steps)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(pkg, w.params)

@@ -257,8 +257,8 @@ void swl_batch2d_runner_destroy(swl_batch2d_runner *r);
 /* options: "graphs" (default 1), "autotune" (1), "grid" (0 = tuned / one block per tile), "chunk" (32 steps) */
 int swl_batch2d_runner_set_option(swl_batch2d_runner *r, const char *name, int value);
 int swl_batch2d_run(swl_batch2d_runner *r, swl_batch_shot *shots, int n);
-/* out[10]: chosen grid (-1 untuned, 0 = one block per tile), graph captures, graph replays, direct chunks,
-   chunks, autotuner candidate ms x4, chunk length */
+/* out[11]: chosen grid (-1 untuned, 0 = one block per tile), graph captures, graph replays, direct chunks,
+   chunks, autotuner candidate ms x4, chunk length, failed captures (then launched directly) */
 int swl_batch2d_runner_stats(const swl_batch2d_runner *r, double *out);
 
 /* Session-level batch over swl_sim2d sessions created on the runner's device (not owned): advance steps

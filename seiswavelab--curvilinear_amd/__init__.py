@@ -7,4 +7,4 @@ from .params import (Absorber, AbsorbingBoundarySettings3D, BackendKind, CfsPmlS
                      LayerDepthMode3D, MaterialLayer3D, Parameters, Precision, ReceiverArrayKind3D,
                      ReceiverArraySettings3D, SimulationParameters, SourceKind, SourceMechanism, SurfaceGrid3D)
 from .simulation import Batch2D, Receiver, ReceiverTrace3D, Simulation, Simulation3D, writeReceivers
-from . import native
+from . import native, workloads

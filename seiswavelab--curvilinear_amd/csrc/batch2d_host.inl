@@ -95,7 +95,7 @@ struct Runner2D {
     std::vector<int> candidates;
     std::vector<double> candidateMs;
     int tuneNext = 0, chosen = -1;
-    long long captures = 0, replays = 0, directChunks = 0, chunks = 0;
+    long long captures = 0, replays = 0, directChunks = 0, chunks = 0, captureFailures = 0;
     hipEvent_t t0{}, t1{};
     BatchTyped<double> b64;
     BatchTyped<float> b32;
@@ -267,23 +267,56 @@ struct Runner2D {
             // ---- launch (graph replay when the structure repeats)
             const bool timing = autotune && chosen < 0 && fixedGrid == 0 && K == chunk;
             if (timing) check(hipEventRecord(t0, stream), "batch timing");
+            bool launched = false;
             if (useGraphs) {
                 if (B.signature != B.cachedSignature || !B.graph) {
                     if (B.graph) (void)hipGraphExecDestroy(B.graph), B.graph = nullptr;
+                    B.cachedSignature.clear();
+                    // relaxed mode: sessions being built on other threads (allocations, uploads on their own
+                    // streams) do not invalidate this capture
                     hipGraph_t g{};
-                    check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "batch capture");
-                    enqueue(B, n);
-                    check(hipStreamEndCapture(stream, &g), "batch capture end");
-                    const hipError_t ie = hipGraphInstantiate(&B.graph, g, nullptr, nullptr, 0);
-                    (void)hipGraphDestroy(g);
-                    check(ie, "batch graph instantiate");
-                    B.cachedSignature = B.signature;
-                    ++captures;
+                    std::unique_lock<std::mutex> setup(device_setup_mutex());
+                    hipError_t ce = hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed);
+                    if (ce == hipSuccess) {
+                        try {
+                            enqueue(B, n);
+                        } catch (const std::exception &) {
+                            ce = hipErrorStreamCaptureInvalidated;
+                        }
+                        const hipError_t ee = hipStreamEndCapture(stream, &g);
+                        if (ce == hipSuccess) ce = ee;
+                    }
+                    setup.unlock();
+                    if (ce == hipSuccess) ce = hipGraphInstantiate(&B.graph, g, nullptr, nullptr, 0);
+                    if (g) (void)hipGraphDestroy(g);
+                    if (ce == hipSuccess) {
+                        B.cachedSignature = B.signature;
+                        ++captures;
+                    } else { // capture unavailable: launch directly from now on (same kernels, same results)
+                        (void)hipGetLastError();
+                        if (B.graph) (void)hipGraphExecDestroy(B.graph), B.graph = nullptr;
+                        useGraphs = 0, ++captureFailures;
+                        // continue on a fresh stream: the failed capture may leave state on the old one
+                        hipStream_t fresh{};
+                        check(hipStreamCreateWithFlags(&fresh, hipStreamNonBlocking), "batch stream");
+                        (void)hipStreamDestroy(stream);
+                        stream = fresh;
+                        check(hipMemcpyAsync(B.args.ptr, B.hargs.ptr, argBytes, hipMemcpyHostToDevice, stream),
+                              "batch arguments");
+                        check(hipMemcpyAsync(B.finals.ptr, B.hfinals.ptr, finBytes, hipMemcpyHostToDevice, stream),
+                              "batch arguments");
+                        check(hipMemcpyAsync(B.begin.ptr, B.hbegin.ptr, beginBytes, hipMemcpyHostToDevice, stream),
+                              "batch counters");
+                    }
                 } else {
                     ++replays;
                 }
-                check(hipGraphLaunch(B.graph, stream), "batch graph launch");
-            } else {
+                if (B.graph) {
+                    check(hipGraphLaunch(B.graph, stream), "batch graph launch");
+                    launched = true;
+                }
+            }
+            if (!launched) {
                 enqueue(B, n);
                 ++directChunks;
             }
@@ -385,6 +418,7 @@ int swl_batch2d_runner_stats(const swl_batch2d_runner *r, double *out) {
     out[1] = (double)x.captures, out[2] = (double)x.replays, out[3] = (double)x.directChunks, out[4] = (double)x.chunks;
     for (int c = 0; c < 4; ++c) out[5 + c] = c < (int)x.candidateMs.size() ? x.candidateMs[c] : -1;
     out[9] = x.chunk;
+    out[10] = (double)x.captureFailures;
     return 0;
 }
 } // extern "C"

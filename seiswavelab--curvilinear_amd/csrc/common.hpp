@@ -4,12 +4,17 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <utility>
 
 namespace swl {
 std::string &last_error();
+// Held while an engine is built (allocations, synchronous uploads) and while a stream is captured into a
+// hipGraph, so that synchronous HIP calls of a session being built on another thread cannot invalidate
+// a capture in progress.
+std::mutex &device_setup_mutex();
 
 inline void check(hipError_t e, const char *action) {
     if (e != hipSuccess) throw std::runtime_error(std::string("HIP ") + action + ": " + hipGetErrorString(e));

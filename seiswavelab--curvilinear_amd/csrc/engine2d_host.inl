@@ -525,6 +525,7 @@ extern "C" {
 int swl_engine2d_create(const swl_mesh2d_desc *mesh, const int64_t *receiver_index, int receivers, int device,
                         swl_engine2d **out) {
     return swl::guarded([&] {
+        std::lock_guard<std::mutex> setup(swl::device_setup_mutex());
         swl::require(mesh && out && receivers >= 0 && (receivers == 0 || receiver_index), "invalid 2D engine arguments");
         auto e = std::make_unique<swl_engine2d>();
         e->precision = mesh->precision;
@@ -533,7 +534,10 @@ int swl_engine2d_create(const swl_mesh2d_desc *mesh, const int64_t *receiver_ind
         *out = e.release();
     });
 }
-void swl_engine2d_destroy(swl_engine2d *e) { delete e; }
+void swl_engine2d_destroy(swl_engine2d *e) {
+    std::lock_guard<std::mutex> setup(swl::device_setup_mutex()); // frees synchronise the device
+    delete e;
+}
 int swl_engine2d_step(swl_engine2d *e, double time, double dt, double frequency, double scale, int step) {
     return swl::guarded([&] { e->with([&](auto &x) { x.step(time, dt, frequency, scale, step); }); });
 }

@@ -404,6 +404,7 @@ struct swl_engine3d {
 extern "C" {
 int swl_engine3d_create(const swl_mesh3d_desc *d, const int64_t *rec, int n, int device, swl_engine3d **out) {
     return swl::guarded([&] {
+        std::lock_guard<std::mutex> setup(swl::device_setup_mutex());
         swl::require(d && out && (n == 0 || rec), "invalid 3D engine arguments");
         auto e = std::make_unique<swl_engine3d>();
         e->precision = d->precision;
@@ -412,7 +413,10 @@ int swl_engine3d_create(const swl_mesh3d_desc *d, const int64_t *rec, int n, int
         *out = e.release();
     });
 }
-void swl_engine3d_destroy(swl_engine3d *e) { delete e; }
+void swl_engine3d_destroy(swl_engine3d *e) {
+    std::lock_guard<std::mutex> setup(swl::device_setup_mutex());
+    delete e;
+}
 int swl_engine3d_step(swl_engine3d *e, double time, double dt, double frequency, double scale, int step) {
     return swl::guarded([&] { e->with([&](auto &x) { x.step(time, dt, frequency, scale, step); }); });
 }

@@ -18,6 +18,10 @@ std::string &last_error() {
     thread_local std::string message;
     return message;
 }
+std::mutex &device_setup_mutex() {
+    static std::mutex m;
+    return m;
+}
 } // namespace swl
 
 namespace {

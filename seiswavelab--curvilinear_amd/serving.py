@@ -1,0 +1,197 @@
+"""Continuous-batching shot server on one GPU.
+
+Requests (2D shots) arrive over time.  The server admits a request when its estimated device
+footprint fits the HBM budget, builds its session (FP64 model construction + upload) on a pool of
+builder threads, and adds it to the running batch between chunks of steps, so a request that arrives
+while others are on the device joins them as soon as memory allows; finished shots leave the batch
+and return their traces.  Every shot keeps the single-session semantics (its result is the same alone
+and in the mix).  Latency is completion time minus arrival time.
+
+No counterpart in the reference (its sessions step one shot each, src/two_d/AsyncSimulation2D.cpp
+runs one session per background worker); its per-session memory model (MemoryBudget2D.hpp:12-38) is
+the basis of the footprint estimate here.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import copy
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from .params import Absorber, BackendKind, Parameters, Precision
+from .simulation import Batch2D, Simulation
+
+
+def footprint_bytes(p: Parameters) -> int:
+    """Device bytes of a session: three rotating state buffers + accumulator (GSLS: 6 + 4L fields),
+    the mesh arrays, PML memories and row padding (the engine's aligned layout)."""
+    w = min(max(p.spongeX, 1), max(1, p.nx // 3)) if p.absorbing else 0
+    wz = min(max(p.spongeZ, 1), max(1, p.nz // 3)) if p.absorbing else 0
+    nx, nz = p.nx, p.nz
+    if p.externalAbsorption and p.absorbing:
+        nx, nz = nx + 2 * w, nz + wz
+    ld = (nx + 6 + 61 + 63) // 64 * 64
+    count = ld * (nz + 6)
+    mech = p.attenuation.mechanisms if p.attenuation.enabled else 0
+    fields = 6 + 4 * mech if mech else 5
+    word = 4 if p.precision == Precision.Float32 else 8
+    mesh_arrays = 11 + 2 * mech
+    pml = 4 * (2 * w * nz + wz * nx) * 4 * word if p.absorber == Absorber.CfsPml else 0
+    return int(count * word * (4 * fields + mesh_arrays) + pml + (16 << 20))
+
+
+@dataclass
+class Result:
+    index: int
+    arrival: float
+    admitted: float = 0.0
+    started: float = 0.0
+    finished: float = 0.0
+    steps: int = 0
+    nodes: int = 0
+    footprint: int = 0
+    error: str = ""
+    traces: Optional[Dict[str, np.ndarray]] = None
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+@dataclass
+class ServeReport:
+    results: List[Result]
+    wall: float
+    offered_node_steps: float
+    stats: dict = field(default_factory=dict)
+
+    def latencies(self):
+        return np.array([r.latency for r in self.results if not r.error])
+
+    def summary(self) -> dict:
+        lat = self.latencies()
+        done = [r for r in self.results if not r.error]
+        return {"requests": len(self.results), "completed": len(done),
+                "shots_per_s": round(len(done) / self.wall, 4) if self.wall > 0 else 0.0,
+                "p50_s": round(float(np.percentile(lat, 50)), 4) if len(lat) else None,
+                "p99_s": round(float(np.percentile(lat, 99)), 4) if len(lat) else None,
+                "mean_s": round(float(lat.mean()), 4) if len(lat) else None,
+                "makespan_s": round(self.wall, 4)}
+
+
+class Server:
+    """budget_bytes: HBM admission budget; chunk: steps between admission / completion checks."""
+
+    def __init__(self, device: int = 0, budget_bytes: float = 200e9, chunk: int = 32, builders: int = 6,
+                 keep_traces: bool = True):
+        self.device, self.budget, self.chunk, self.builders = device, budget_bytes, chunk, builders
+        self.keep_traces = keep_traces
+
+    @staticmethod
+    def _build(p: Parameters, device: int):
+        s = Simulation()
+        ok, err = s.initialize(p, BackendKind.Hip, device)
+        return s, ("" if ok else err)
+
+    def run(self, requests) -> ServeReport:
+        """requests: iterable of (index, arrival_s, Parameters); each shot runs its full step count."""
+        reqs = sorted(list(requests), key=lambda r: r[1])
+        results = {i: Result(i, a) for i, a, _ in reqs}
+        batch = Batch2D(self.device)
+        pool = cf.ThreadPoolExecutor(max_workers=self.builders)
+        queue = list(reqs)  # not yet admitted, in arrival order
+        building: Dict[int, cf.Future] = {}
+        running: Dict[int, Simulation] = {}
+        used, self.max_running, self.max_used = 0, 0, 0
+        offered = 0.0
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0  # noqa: E731
+        while queue or building or running:
+            # admission: arrived requests, FIFO, while the footprint fits the budget
+            while queue and queue[0][1] <= now():
+                i, arrival, p = queue[0]
+                need = footprint_bytes(p)
+                if running or building:
+                    if used + need > self.budget:
+                        break
+                queue.pop(0)
+                used += need
+                self.max_used = max(self.max_used, used)
+                r = results[i]
+                r.admitted, r.footprint = now(), need
+                building[i] = pool.submit(self._build, p, self.device)
+            # built sessions join the batch between chunks
+            for i in [i for i, f in building.items() if f.done()]:
+                s, err = building.pop(i).result()
+                r = results[i]
+                if err:
+                    r.error, r.finished = err, now()
+                    used -= r.footprint
+                    continue
+                L = s.layout()
+                r.nodes = (L["nx"] - 6) * (L["nz"] - 6)
+                r.steps = s.totalSteps()
+                offered += r.nodes * r.steps
+                r.started = now()
+                batch.add(s)
+                running[i] = s
+                self.max_running = max(self.max_running, len(running))
+            if running:
+                # every member advances up to `chunk` steps and stops at its own total
+                ok, err = batch.advance(self.chunk)
+                if not ok:
+                    raise RuntimeError(err)
+                for i in [i for i, s in running.items() if s.step() >= s.totalSteps() or s.fault()]:
+                    s = running.pop(i)
+                    r = results[i]
+                    r.finished = now()
+                    r.error = s.fault()
+                    if self.keep_traces:
+                        r.traces = s.receiver_arrays()
+                    batch.remove(s)
+                    s.close()
+                    used -= r.footprint
+            elif building:
+                time.sleep(0.0005)
+            elif queue:
+                time.sleep(max(0.0, min(0.002, queue[0][1] - now())))
+        wall = now()
+        stats = batch.stats()
+        batch.close()
+        pool.shutdown(wait=True)
+        return ServeReport([results[i] for i, _, _ in reqs], wall, offered, stats)
+
+
+def serve_sequential(requests, device: int = 0, precision=None, keep_traces: bool = True) -> ServeReport:
+    """The plain path: one shot at a time in arrival order (each starts when it has arrived and the
+    previous one has finished), optionally forcing the precision (e.g. FP64)."""
+    reqs = sorted(list(requests), key=lambda r: r[1])
+    results = []
+    t0 = time.perf_counter()
+    now = lambda: time.perf_counter() - t0  # noqa: E731
+    for i, arrival, p in reqs:
+        if precision is not None:
+            p = copy.copy(p)
+            p.precision = precision
+        while now() < arrival:
+            time.sleep(min(0.002, arrival - now()))
+        r = Result(i, arrival, admitted=now())
+        s = Simulation()
+        ok, err = s.initialize(p, BackendKind.Hip, device)
+        if not ok:
+            r.error, r.finished = err, now()
+            results.append(r)
+            continue
+        r.started = now()
+        r.steps = s.totalSteps()
+        ok, err = s.advance(r.steps)
+        r.finished = now()
+        r.error = "" if ok else err
+        if keep_traces:
+            r.traces = s.receiver_arrays()
+        s.close()
+        results.append(r)
+    return ServeReport(results, now(), 0.0)

@@ -78,14 +78,14 @@ class Heterogeneous2D:
 
 
 def grf_terrain_5001x2001(seed: int = 2, precision: Precision = Precision.Float32, nx: int = 5001, nz: int = 2001,
-            steps: int = 4000, source_z: int = 5) -> Heterogeneous2D:
+            steps: int = 4000, source_z: int = 5, topo_rms: float = 25.0) -> Heterogeneous2D:
     """5001x2001 at h=2: GRF topography (60 m correlation, 25 m RMS), three layers each perturbed by
     independent 5 % RMS GRFs (20 m correlation) on Vp and Vs with Vp/Vs >= 1.6, 30 Hz Ricker
     explosive moment, 30-node external CFS-PML, 1001 surface receivers every 10 m.  The explosive
     source sits 5 nodes deep (10 m), the shallowest the reference accepts."""
     rng = np.random.default_rng(seed)
     h = 2.0
-    topo = grf_1d(nx, h, 60.0, 25.0, rng)
+    topo = grf_1d(nx, h, 60.0, topo_rms, rng)
     x = np.arange(nx) * h
     bvp, bvs, brho = layered_base(nz, h, THREE_LAYERS)
     dp = grf_2d(nz, nx, h, 20.0, 0.05, rng)
@@ -173,3 +173,28 @@ def gsls_shot(rng: np.random.Generator, precision: Precision = Precision.Float32
 def gsls_batch(seed: int = 3, shots: int = 64, precision: Precision = Precision.Float32, steps: int = 3000):
     rng = np.random.default_rng(seed)
     return [gsls_shot(rng, precision, steps) for _ in range(shots)]
+
+
+
+def serving_requests(seed: int = 5, count: int = 48, rate: float = 20.0, steps: int = 500,
+                     precision: Precision = Precision.Float32):
+    """Poisson stream (rate req/s) of 2D shots: nx ~ logU[200, 3000], nz ~ logU[100, 1200]; half elastic, half
+    GSLS-3 (Qp ~ U[20, 80], Qs = Qp/2, band 6-90 Hz, reference 30 Hz); models drawn as the 5001x2001 workload
+    (GRF terrain, 5 % RMS GRF velocity perturbation, explosive source 5 nodes deep, 30-node external
+    CFS-PML, surface receivers every 5 nodes); terrain RMS min(25 m, 5 % of the depth).  Yields
+    (index, arrival_s, Parameters)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    for i in range(count):
+        t += float(rng.exponential(1.0 / rate))
+        nx = int(round(math.exp(rng.uniform(math.log(200), math.log(3000)))))
+        nz = int(round(math.exp(rng.uniform(math.log(100), math.log(1200)))))
+        gsls = bool(rng.random() < 0.5)
+        qp = float(rng.uniform(20, 80))
+        w = grf_terrain_5001x2001(seed=seed * 1000 + i, precision=precision, nx=nx, nz=nz, steps=steps,
+                                  topo_rms=min(25.0, 0.05 * nz * 2.0))
+        p = w.params
+        if gsls:
+            p.attenuation = GslsSettings(enabled=True, mechanisms=3, minFrequency=6.0, maxFrequency=90.0,
+                                         referenceFrequency=30.0, qp=qp, qs=qp / 2)
+        yield i, t, p
