@@ -81,7 +81,8 @@ template <class R> struct Stage3 {
     int kind;         // 0 first, 1 middle, 2 final (+ inspection)
     const int *status;
     Partial3 *partials;
-    int ntx, nty, ntiles;
+    int ntx, nty, ntiles; // tiles of 64x4 nodes per plane row / column, tiles of all planes
+    int t0, nband;        // this launch covers tiles [t0, t0 + nband) (a band of k planes)
 };
 
 __device__ inline int tile_of_block(int b, int n) { // XCD-contiguous band of tiles
@@ -207,11 +208,125 @@ template <class R> __device__ inline R mat(const DMesh3<R> &m, const R *a, long 
 }
 
 // One RK stage over every physical node (one node per lane, 64 x 4 nodes per block, one k plane).
-template <class R, int DX, int DY, int DZ> __global__ __launch_bounds__(THREADS) void stage3_kernel(Stage3<R> A) {
+// Depth bands of one stage: SURFACE = planes k = 3..6 (closure, reduced stencils, traction image; the
+// general per-node path below), TERRAIN = deeper planes with stored metrics, FLAT = planes with identity
+// metrics and J = 1 (below the terrain transition).  The deep bands evaluate the same expressions of
+// PointKernels3D.hpp:92-112 field by field, so far fewer values are live at once.
+constexpr int SURFACE = 0, TERRAIN = 1, FLAT = 2;
+
+template <class R, int DX, int DY, int DZ, int BAND>
+__device__ __forceinline__ void deep_node(const Stage3<R> &A, long long q, int i, int j, int k, Partial3 &p) {
+    const DMesh3<R> &m = A.m;
+    const size_t cnt = A.count;
+    const long long sy = m.ld, sz = m.plane;
+    const R *w = A.in;
+    const R h = m.h, hinv = m.hinv;
+    const R lambda = mat(m, m.lam, q, k), mu = mat(m, m.mu, q, k), density = mat(m, m.rho, q, k);
+    R mt[9];
+    if constexpr (BAND == TERRAIN) {
+#pragma unroll
+        for (int a = 0; a < 9; ++a) mt[a] = m.met[(size_t)a * m.mcount + q];
+    }
+    // derivatives of field f along xi, eta, zeta (DRP) and the requested Cartesian components
+    auto deriv = [&](int f, R d[3]) {
+        const R *v = w + (size_t)f * cnt;
+        d[0] = drp<R, DX>(v, q, 1, h, hinv);
+        d[1] = drp<R, DY>(v, q, sy, h, hinv);
+        d[2] = drp<R, DZ>(v, q, sz, h, hinv);
+    };
+    auto comp = [&](const R d[3], int a) -> R {
+        if constexpr (BAND == FLAT) return d[a]; // identity metrics: m[a]*d0 + m[3+a]*d1 + m[6+a]*d2 = d[a]
+        else return mt[a] * d[0] + mt[3 + a] * d[1] + mt[6 + a] * d[2];
+    };
+    R kv[9], d[3];
+    // velocity gradient -> stress rates
+    R g[3][3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        deriv(f, d);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) g[f][a] = comp(d, a);
+    }
+    const R div = g[0][0] + g[1][1] + g[2][2];
+    kv[3] = lambda * div + 2 * mu * g[0][0];
+    kv[4] = lambda * div + 2 * mu * g[1][1];
+    kv[5] = lambda * div + 2 * mu * g[2][2];
+    kv[6] = mu * (g[0][1] + g[1][0]);
+    kv[7] = mu * (g[0][2] + g[2][0]);
+    kv[8] = mu * (g[1][2] + g[2][1]);
+    // momentum: ((g30 + g61) + g72), ((g60 + g41) + g82), ((g70 + g81) + g52), each / density
+    deriv(6, d);
+    const R g60 = comp(d, 0), g61 = comp(d, 1);
+    deriv(3, d);
+    const R m0 = comp(d, 0) + g61;
+    deriv(4, d);
+    const R m1 = g60 + comp(d, 1);
+    deriv(7, d);
+    const R g70 = comp(d, 0);
+    kv[0] = (m0 + comp(d, 2)) / density;
+    deriv(8, d);
+    const R m2 = g70 + comp(d, 1);
+    kv[1] = (m1 + comp(d, 2)) / density;
+    deriv(5, d);
+    kv[2] = (m2 + comp(d, 2)) / density;
+    if (m.si1 >= 0 && i >= m.si0 && i <= m.si1 && j >= m.sj0 && j <= m.sj1 && k >= m.sk0 && k <= m.sk1) {
+        for (int a = 0; a < 3; ++a)
+            kv[a] += A.forcing * (m.grad[0] ? m.grad[a][q] : m.src[q] * m.dir[a]) * (m.rho_global / density);
+    }
+    const R damp = m.damp ? m.damp[q] : (m.dax[i] + m.day[j]) + m.daz[k];
+    R nxt[9];
+#pragma unroll
+    for (int f = 0; f < 9; ++f) {
+        const size_t o = (size_t)f * cnt + q;
+        const R kf = kv[f] - damp * w[o];
+        if (A.kind == 2) {
+            nxt[f] = A.acc[o] + A.wy * kf;
+        } else {
+            const R b = A.base[o];
+            nxt[f] = b + A.w * kf;
+            A.acc[o] = (A.kind == 0 ? b : A.acc[o]) + A.wy * kf;
+        }
+        A.out[o] = nxt[f];
+    }
+    if (A.kind == 2) {
+        bool bad = false;
+        for (int f = 0; f < 9; ++f) bad = bad || !isfinite(nxt[f]);
+        const double vx = nxt[0], vy = nxt[1], vz = nxt[2];
+        const double v2 = vx * vx + vy * vy + vz * vz, sv = sqrt(v2);
+        p.velocity = sv;
+        p.peak[0] = fabs(vz), p.peak[1] = fabs(vx), p.peak[2] = fabs(vy), p.peak[3] = sv;
+        const double sxx = nxt[3], syy = nxt[4], szz = nxt[5], sxy = nxt[6], sxz = nxt[7], syz = nxt[8];
+        p.peak[4] = fabs((sxx + syy + szz) / 3.0);
+        const double dl = lambda, dm = mu, dr = density, mean = (sxx + syy + szz) / 3;
+        double dev = 0;
+        const double tt[3] = {sxx - mean, syy - mean, szz - mean};
+        for (int c = 0; c < 3; ++c) dev += tt[c] * tt[c];
+        dev += 2 * sxy * sxy, dev += 2 * sxz * sxz, dev += 2 * syz * syz;
+        const double bulk = dl + 2 * dm / 3, hh = (double)h;
+        const double jj = BAND == FLAT ? 1.0 : (double)m.jac[q];
+        p.energy = (0.5 * dr * v2 + dev / (4 * dm) + mean * mean / (2 * bulk)) * jj * hh * hh * hh;
+        if (!isfinite(v2) || !isfinite(p.energy)) bad = true;
+        p.bad = bad;
+    }
+}
+
+template <class R, int DX, int DY, int DZ, int BAND>
+__global__ __launch_bounds__(THREADS) void stage3_kernel(Stage3<R> A) {
     __shared__ Partial3 sh[THREADS];
     if (*A.status) return;
     const DMesh3<R> &m = A.m;
-    const int t = tile_of_block(blockIdx.x, A.ntiles);
+    const int t = A.t0 + tile_of_block(blockIdx.x, A.nband);
+    if constexpr (BAND != SURFACE) {
+        const int per = A.ntx * A.nty, kk = t / per, rem = t % per;
+        const int i = 3 + (rem % A.ntx) * TX + (threadIdx.x % TX), j = 3 + (rem / A.ntx) * TY + threadIdx.x / TX, k = 3 + kk;
+        Partial3 p{};
+        if (i < m.nx - 3 && j < m.ny - 3) deep_node<R, DX, DY, DZ, BAND>(A, at3(m, i, j, k), i, j, k, p);
+        if (A.kind == 2) {
+            block_reduce3(p, sh);
+            if (threadIdx.x == 0) A.partials[t] = sh[0];
+        }
+        return;
+    }
     const int per = A.ntx * A.nty, kk = t / per, rem = t % per;
     const int i = 3 + (rem % A.ntx) * TX + (threadIdx.x % TX), j = 3 + (rem / A.ntx) * TY + threadIdx.x / TX, k = 3 + kk;
     Partial3 p{};

@@ -182,7 +182,7 @@ template <class R> struct Engine3D {
         const int pl = std::max({(nx - 6) * (nz - 6), (ny - 6) * (nz - 6), (nx - 6) * (ny - 6)});
         slice.allocate((size_t)pl * sizeof(float));
         hipFuncAttributes attr{};
-        if (hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&stage3_kernel<R, 0, 0, 0>)) == hipSuccess)
+        if (hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&stage3_kernel<R, 0, 0, 0, FLAT>)) == hipSuccess)
             registers = attr.numRegs;
         check(hipStreamSynchronize(stream), "3D engine setup");
     }
@@ -195,8 +195,17 @@ template <class R> struct Engine3D {
         static const int flags[8][3] = {{0, 0, 0}, {1, 1, 0}, {1, 1, 1}, {0, 0, 1}, {0, 1, 0}, {1, 0, 0}, {1, 0, 1}, {0, 1, 1}};
         return (flags[step % 8][axis] + stage) % 2;
     }
-    template <int DX, int DY, int DZ> void launch(const Stage3<R> &a) {
-        stage3_kernel<R, DX, DY, DZ><<<ntiles, THREADS, 0, stream>>>(a);
+    // planes [3, 7) surface band, [7, flat) terrain band, [max(7, flat), nz - 3) flat band
+    template <int DX, int DY, int DZ> void launch(Stage3<R> a) {
+        const int per = ntx * nty, kEnd = nz - 3, kSurf = std::min(7, kEnd), kFlat = std::clamp(m.flat, kSurf, kEnd);
+        const int bands[3][2] = {{3, kSurf}, {kSurf, kFlat}, {kFlat, kEnd}};
+        for (int b = 0; b < 3; ++b) {
+            if (bands[b][1] <= bands[b][0]) continue;
+            a.t0 = per * (bands[b][0] - 3), a.nband = per * (bands[b][1] - bands[b][0]);
+            if (b == 0) stage3_kernel<R, DX, DY, DZ, SURFACE><<<a.nband, THREADS, 0, stream>>>(a);
+            else if (b == 1) stage3_kernel<R, DX, DY, DZ, TERRAIN><<<a.nband, THREADS, 0, stream>>>(a);
+            else stage3_kernel<R, DX, DY, DZ, FLAT><<<a.nband, THREADS, 0, stream>>>(a);
+        }
     }
     void launch_stage(int dx, int dy, int dz, const Stage3<R> &a) {
         switch (dx * 4 + dy * 2 + dz) {
