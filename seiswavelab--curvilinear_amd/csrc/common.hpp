@@ -59,7 +59,10 @@ struct DeviceBuffer {
         bytes = n;
         if (!n) return;
         check(hipMalloc(&ptr, n), "allocation");
-        check(hipMemset(ptr, 0, n), "clear allocation");
+        // the clear runs on the null stream, which the engines' non-blocking streams do not wait for:
+        // finish it here so a kernel enqueued next on such a stream cannot run before it
+        check(hipMemsetAsync(ptr, 0, n, nullptr), "clear allocation");
+        check(hipStreamSynchronize(nullptr), "clear allocation");
     }
     void upload(const void *host, size_t n) {
         allocate(n);

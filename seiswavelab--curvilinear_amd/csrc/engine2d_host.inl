@@ -415,6 +415,7 @@ template <class R> struct Engine2D {
         DeviceScope scope(device);
         require(f >= 0 && f < fields, "2D snapshot field out of range");
         std::vector<R> v(count);
+        check(hipStreamSynchronize(stream), "2D snapshot"); // the null-stream copy does not wait for it
         check(hipMemcpy(v.data(), S(cur) + (size_t)f * count, count * sizeof(R), hipMemcpyDeviceToHost), "2D snapshot");
         for (int k = 0; k < nz; ++k)
             for (int i = 0; i < nx; ++i) out[(size_t)i + (size_t)nx * k] = v[idx(i, k)];
@@ -423,6 +424,7 @@ template <class R> struct Engine2D {
         DeviceScope scope(device);
         if (!cpml) return;
         std::vector<R> v(cpml);
+        check(hipStreamSynchronize(stream), "2D PML snapshot");
         check(hipMemcpy(v.data(), P(cur), cpml * sizeof(R), hipMemcpyDeviceToHost), "2D PML snapshot");
         for (long long q = 0; q < cpml; ++q) out[q] = v[q];
     }

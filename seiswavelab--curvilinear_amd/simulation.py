@@ -200,7 +200,7 @@ class Simulation:
         return native.lib().swl_sim2d_fault(self._h).decode()
 
     def set_option(self, name: str, value: int):
-        """Engine tuning switch (currently "nontemporal_stores")."""
+        """Engine tuning switch: "nontemporal_stores" or "stored_metrics"."""
         self._need()
         native.call("swl_sim2d_set_option", self._h, name.encode(), int(value))
 

@@ -77,6 +77,19 @@ def test_reset_is_bitwise_reproducible_and_chunking_invariant_3d():
     assert np.array_equal(f0, s.field(2))
 
 
+def test_first_download_is_complete_3d():
+    """The first field download allocates its staging buffer; the allocation's clear must be finished
+    before the conversion kernel on the engine's stream writes into it (it once raced and zeroed part of
+    the first field read)."""
+    p = case("dem_layers_moment")
+    for _ in range(3):
+        s = _gpu(p, 0)
+        assert s.advance(p.maxSteps)[0]
+        first = s.field(0)
+        assert np.array_equal(first, s.field(0))
+        s.close()
+
+
 def test_display_frame_planes_3d():
     p = case("analytic_sponge_force")
     s = _gpu(p, 0)
