@@ -240,10 +240,14 @@ size_t swl_params3d_size(void);
 
 /* ---------------- 2D multi-shot batch (no counterpart in the reference) ----------------
    The reference steps one shot per session (CudaEngine2D.cu:143-152).  A runner steps many 2D engines
-   together: per RK stage one launch covers the tiles of every shot that shares a precision, a GSLS
+   together in one of two modes.  Lanes (default, option "fused" 0): every shot keeps its own specialised
+   stage kernels; shots are placed longest-first on the least-loaded of `lanes` streams and enqueued
+   `lane_steps` steps at a time round-robin, so small shots' launches overlap on the device.  Fused
+   ("fused" 1): per RK stage one launch covers the tiles of every shot that shares a precision, a GSLS
    mechanism count and the stencil direction pair; chunks of steps are captured into hipGraphs and
-   replayed; an autotuner picks the persistent grid size.  Each shot keeps the semantics of
-   swl_engine2d_run (device-side fault rule, receiver gather, its own dt / frequency / step number). */
+   replayed; an autotuner picks the persistent grid size.  Either way each shot keeps the semantics of
+   swl_engine2d_run (device-side fault rule, receiver gather, its own dt / frequency / step number) and
+   its results are bit-identical to running it alone. */
 typedef struct swl_batch_shot {
     swl_engine2d *engine;
     int first_step, count;                        /* steps first_step .. first_step + count - 1 */
@@ -254,11 +258,13 @@ typedef struct swl_batch_shot {
 typedef struct swl_batch2d_runner swl_batch2d_runner;
 int swl_batch2d_runner_create(int device, swl_batch2d_runner **out);
 void swl_batch2d_runner_destroy(swl_batch2d_runner *r);
-/* options: "graphs" (default 1), "autotune" (1), "grid" (0 = tuned / one block per tile), "chunk" (32 steps) */
+/* options: "fused" (default 0 = lanes), "lanes" (2 streams), "lane_steps" (8); fused mode: "graphs" (1),
+   "autotune" (1), "grid" (0 = tuned / one block per tile), "chunk" (32 steps) */
 int swl_batch2d_runner_set_option(swl_batch2d_runner *r, const char *name, int value);
 int swl_batch2d_run(swl_batch2d_runner *r, swl_batch_shot *shots, int n);
-/* out[11]: chosen grid (-1 untuned, 0 = one block per tile), graph captures, graph replays, direct chunks,
-   chunks, autotuner candidate ms x4, chunk length, failed captures (then launched directly) */
+/* out[14]: chosen grid (-1 untuned, 0 = one block per tile), graph captures, graph replays, direct chunks,
+   chunks, autotuner candidate ms x4, chunk length, failed captures (then launched directly), fused (0/1),
+   lanes, lane runs */
 int swl_batch2d_runner_stats(const swl_batch2d_runner *r, double *out);
 
 /* Session-level batch over swl_sim2d sessions created on the runner's device (not owned): advance steps

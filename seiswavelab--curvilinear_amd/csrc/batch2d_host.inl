@@ -89,6 +89,15 @@ template <class R> struct BatchTyped {
 struct Runner2D {
     int device = 0;
     hipStream_t stream{};
+    // fused = 1: the batch kernels above (one launch per stage over every shot's tiles);
+    // fused = 0: each shot keeps its own specialised stage kernels and the shots are spread over `lanes`
+    // streams (longest-first onto the least-loaded lane), so small shots' launches overlap on the device.
+    // Two lanes measured best for shots of ~0.1-1 M nodes: two shots' working sets (~100 MB each for
+    // GSLS-3 FP32) still fit the 256 MB Infinity Cache together, four do not.
+    int fused = 0, lanes = 2, laneSteps = 8;
+    std::vector<hipStream_t> laneStreams;
+    PinnedBuffer verdicts;
+    long long laneRuns = 0;
     int chunk = 32, useGraphs = 1, autotune = 1, fixedGrid = 0;
     int cus = 256;
     // autotuner: candidate persistent grids (0 = one block per work item)
@@ -111,6 +120,7 @@ struct Runner2D {
         candidates = {0, 8 * cus, 4 * cus, 2 * cus};
     }
     ~Runner2D() {
+        for (auto s : laneStreams) (void)hipStreamDestroy(s);
         if (t0) (void)hipEventDestroy(t0);
         if (t1) (void)hipEventDestroy(t1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -364,8 +374,84 @@ struct Runner2D {
         }
     }
 
+    // Lanes: every shot runs its own engine's kernels (as swl_engine2d_run would), enqueued `laneSteps`
+    // steps at a time round-robin over the lanes so that every lane's hardware queue is fed from the start.
+    void run_lanes(swl_batch_shot *shots, int n) {
+        while ((int)laneStreams.size() < lanes) {
+            hipStream_t s{};
+            check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "batch lane stream");
+            laneStreams.push_back(s);
+        }
+        struct Item {
+            swl_batch_shot *shot;
+            double cost;
+        };
+        std::vector<Item> items;
+        for (int i = 0; i < n; ++i) {
+            auto &sh = shots[i];
+            sh.result.accepted = 0, sh.result.faulted = 0, sh.result.peak_energy = sh.peak_energy;
+            if (sh.count <= 0) continue;
+            double cost = 0;
+            sh.engine->with([&](auto &e) {
+                require(e.device == device, "batch shots must live on the runner's device");
+                e.reserve_rows(sh.count); // allocations synchronize: all of them before anything is enqueued
+                cost = e.step_bytes() * sh.count;
+            });
+            items.push_back({&sh, cost});
+        }
+        if (items.empty()) return;
+        std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) { return a.cost > b.cost; });
+        std::vector<std::vector<swl_batch_shot *>> plan(lanes);
+        std::vector<double> load(lanes, 0.0);
+        for (const auto &it : items) {
+            const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            plan[l].push_back(it.shot), load[l] += it.cost;
+        }
+        verdicts.reserve(items.size() * 16);
+        using V = Engine2D<double>::Verdict;
+        static_assert(sizeof(V) == sizeof(Engine2D<float>::Verdict), "verdict layout");
+        V *vs = static_cast<V *>(verdicts.ptr);
+        std::vector<std::pair<swl_batch_shot *, V *>> order;
+        for (int l = 0; l < lanes; ++l)
+            for (auto *sh : plan[l]) order.push_back({sh, vs + order.size()});
+        // per lane: index of the current shot in plan[l] and the next step of that shot
+        std::vector<size_t> at(lanes, 0);
+        std::vector<int> next(lanes, 0);
+        std::vector<size_t> base(lanes, 0);
+        for (int l = 1; l < lanes; ++l) base[l] = base[l - 1] + plan[l - 1].size();
+        for (bool any = true; any;) {
+            any = false;
+            for (int l = 0; l < lanes; ++l) {
+                if (at[l] >= plan[l].size()) continue;
+                any = true;
+                swl_batch_shot *sh = plan[l][at[l]];
+                V *v = vs + base[l] + at[l];
+                const hipStream_t s = laneStreams[l];
+                const int j0 = next[l], j1 = std::min(sh->count, j0 + laneSteps);
+                sh->engine->with([&](auto &e) {
+                    if (j0 == 0) e.run_begin(s, sh->peak_energy);
+                    e.run_steps(s, sh->first_step, j0, j1, sh->dt, sh->frequency, sh->scale);
+                    if (j1 == sh->count) e.run_end(s, reinterpret_cast<typename std::decay_t<decltype(e)>::Verdict *>(v));
+                });
+                next[l] = j1;
+                if (j1 == sh->count) ++at[l], next[l] = 0;
+            }
+        }
+        for (auto s : laneStreams) check(hipStreamSynchronize(s), "batch lanes");
+        for (auto &[sh, v] : order)
+            sh->engine->with([&](auto &e) {
+                e.run_finish(*reinterpret_cast<const typename std::decay_t<decltype(e)>::Verdict *>(v), sh->diag,
+                             sh->samples, sh->result);
+            });
+        ++laneRuns, ++chunks;
+    }
+
     void run(swl_batch_shot *shots, int n) {
         DeviceScope scope(device);
+        if (!fused) {
+            run_lanes(shots, n);
+            return;
+        }
         std::vector<Engine2D<double> *> e64;
         std::vector<Engine2D<float> *> e32;
         std::vector<swl_batch_shot *> s64, s32;
@@ -406,6 +492,15 @@ int swl_batch2d_runner_set_option(swl_batch2d_runner *r, const char *name, int v
         else if (key == "autotune") x.autotune = value != 0, x.chosen = -1, x.tuneNext = 0, x.candidateMs.clear();
         else if (key == "grid") x.fixedGrid = std::max(0, value);
         else if (key == "chunk") x.chunk = std::clamp(value, 1, 4096);
+        else if (key == "fused") x.fused = value != 0;
+        else if (key == "lanes") {
+            swl::require(value >= 1 && value <= 16, "lanes must be in 1..16");
+            if (value != x.lanes) {
+                for (auto s : x.laneStreams) (void)hipStreamDestroy(s);
+                x.laneStreams.clear();
+                x.lanes = value;
+            }
+        } else if (key == "lane_steps") x.laneSteps = std::clamp(value, 1, 4096);
         else throw std::runtime_error("unknown batch option: " + key);
     });
 }
@@ -419,6 +514,7 @@ int swl_batch2d_runner_stats(const swl_batch2d_runner *r, double *out) {
     for (int c = 0; c < 4; ++c) out[5 + c] = c < (int)x.candidateMs.size() ? x.candidateMs[c] : -1;
     out[9] = x.chunk;
     out[10] = (double)x.captureFailures;
+    out[11] = x.fused, out[12] = x.lanes, out[13] = (double)x.laneRuns;
     return 0;
 }
 } // extern "C"

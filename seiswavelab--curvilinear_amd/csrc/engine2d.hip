@@ -750,6 +750,11 @@ template <class R> __global__ __launch_bounds__(THREADS) void batch_final_kernel
     finalize_body(A);
 }
 // zeroes the per-shot run counters at the start of a captured chunk (no memset nodes in graphs)
+// start of a device-resident run: counters cleared and the peak seeded from a kernel argument (no host
+// memory is read after the launch returns)
+__global__ void run_begin_kernel(int *status, int *accepted, double *peak, double peak0) {
+    if (threadIdx.x == 0) *status = 0, *accepted = 0, *peak = peak0;
+}
 __global__ void batch_begin_kernel(int *const *status, int *const *accepted, double *const *peak, const double *peak0,
                                    int n) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;

@@ -450,26 +450,37 @@ template <class R> struct Engine2D {
     }
 
     // Device-resident stepping with the reference's inspection / fault rule / sampling per step.
-    void run(int first, int count_steps, double dt, double frequency, double scale, double peak0, double *diag_out,
-             double *samples_out, swl_run2d &res) {
-        DeviceScope scope(device);
-        require(count_steps >= 0, "negative step count");
-        res.accepted = 0, res.faulted = 0, res.peak_energy = peak0;
-        if (!count_steps) return;
-        if (diag_rows < count_steps) {
-            diag.allocate((size_t)count_steps * 4 * sizeof(double));
-            samples.allocate((size_t)count_steps * 2 * std::max(nrec, 1) * sizeof(double));
-            diag_rows = count_steps;
-        }
-        check(hipMemsetAsync(status.ptr, 0, sizeof(int), stream), "2D run status");
-        check(hipMemsetAsync(accepted.ptr, 0, sizeof(int), stream), "2D run counter");
-        check(hipMemcpyAsync(peak.ptr, &peak0, sizeof(double), hipMemcpyHostToDevice, stream), "2D run peak");
-        std::vector<int> states(count_steps + 1);
-        int c = cur;
-        states[0] = c;
-        for (int j = 0; j < count_steps; ++j) {
-            const int s = first + j;
-            const int n = enqueue_step(c, dt, frequency, scale, s);
+    // Device-resident run in three parts, so a runner can interleave many engines over a few streams:
+    // run_begin (counters), run_steps (steps j0..j1-1 of the run, rows j0..j1-1), run_end (verdict copy);
+    // after the stream has finished, run_finish reads the verdict and the rows.  Every part enqueues on
+    // the stream it is given; the engine's own stream is used by run().
+    struct Verdict {
+        int accepted, status;
+        double peak;
+    };
+    struct OnStream { // launches of the engine go to `s` for the scope
+        hipStream_t &ref, old;
+        OnStream(hipStream_t &r, hipStream_t s) : ref(r), old(r) { ref = s; }
+        ~OnStream() { ref = old; }
+    };
+    double step_bytes() const { return (double)count * fields * sizeof(R); } // state bytes, a load-balance weight
+    void reserve_rows(int count_steps) {
+        if (diag_rows >= count_steps) return;
+        diag.allocate((size_t)count_steps * 4 * sizeof(double));
+        samples.allocate((size_t)count_steps * 2 * std::max(nrec, 1) * sizeof(double));
+        diag_rows = count_steps;
+    }
+    void run_begin(hipStream_t s, double peak0) {
+        run_begin_kernel<<<1, 64, 0, s>>>(status.as<int>(), accepted.as<int>(), peak.as<double>(), peak0);
+        check(hipGetLastError(), "2D run start");
+    }
+    void run_steps(hipStream_t s, int first, int j0, int j1, double dt, double frequency, double scale) {
+        require(j1 <= diag_rows, "run rows not reserved");
+        OnStream on(stream, s);
+        int c = (cur + 2 * j0) % 3;
+        for (int j = j0; j < j1; ++j) {
+            const int st = first + j;
+            const int n = enqueue_step(c, dt, frequency, scale, st);
             const bool timed = j % 32 == 0; // finalisation is timed on every 32nd step only (events add gaps)
             if (timed) mark(1, true);
             reduce_kernel<<<nLevel2, THREADS, 0, stream>>>(partials.as<Partial>(), nPartials, level2.as<Partial>(),
@@ -478,35 +489,50 @@ template <class R> struct Engine2D {
             f.partials = level2.as<Partial>(), f.n = nLevel2;
             f.status = status.as<int>(), f.accepted = accepted.as<int>(), f.peak = peak.as<double>();
             f.diag = diag.as<double>(), f.samples = samples.as<double>();
-            f.state = S(n), f.probes = probes.as<long long>(), f.nrec = nrec, f.row = j, f.step = s;
+            f.state = S(n), f.probes = probes.as<long long>(), f.nrec = nrec, f.row = j, f.step = st;
             f.dt = dt, f.frequency = frequency;
             finalize_kernel<R><<<1, THREADS, 0, stream>>>(f);
             if (timed) mark(1, false);
             check(hipGetLastError(), "2D finalize launch");
             c = n;
-            states[j + 1] = c;
         }
-        int acc_steps = 0, st = 0;
-        double pk = peak0;
-        check(hipMemcpyAsync(&acc_steps, accepted.ptr, sizeof(int), hipMemcpyDeviceToHost, stream), "2D run readback");
-        check(hipMemcpyAsync(&st, status.ptr, sizeof(int), hipMemcpyDeviceToHost, stream), "2D run readback");
-        check(hipMemcpyAsync(&pk, peak.ptr, sizeof(double), hipMemcpyDeviceToHost, stream), "2D run readback");
-        check(hipStreamSynchronize(stream), "2D run");
+    }
+    void run_end(hipStream_t s, Verdict *v) {
+        check(hipMemcpyAsync(&v->accepted, accepted.ptr, sizeof(int), hipMemcpyDeviceToHost, s), "2D run readback");
+        check(hipMemcpyAsync(&v->status, status.ptr, sizeof(int), hipMemcpyDeviceToHost, s), "2D run readback");
+        check(hipMemcpyAsync(&v->peak, peak.ptr, sizeof(double), hipMemcpyDeviceToHost, s), "2D run readback");
+    }
+    void run_finish(const Verdict &v, double *diag_out, double *samples_out, swl_run2d &res) {
         collect();
-        if (acc_steps) {
+        const int a = v.accepted;
+        if (a) {
             if (diag_out)
-                check(hipMemcpy(diag_out, diag.ptr, (size_t)acc_steps * 4 * sizeof(double), hipMemcpyDeviceToHost),
+                check(hipMemcpy(diag_out, diag.ptr, (size_t)a * 4 * sizeof(double), hipMemcpyDeviceToHost),
                       "2D run diagnostics");
             if (samples_out && nrec)
-                check(hipMemcpy(samples_out, samples.ptr, (size_t)acc_steps * 2 * nrec * sizeof(double),
-                                hipMemcpyDeviceToHost),
+                check(hipMemcpy(samples_out, samples.ptr, (size_t)a * 2 * nrec * sizeof(double), hipMemcpyDeviceToHost),
                       "2D run samples");
         }
-        res.accepted = acc_steps;
-        res.faulted = st != 0;
-        res.peak_energy = pk;
-        prev = acc_steps ? states[acc_steps - 1] : cur;
-        cur = states[acc_steps];
+        res.accepted = a;
+        res.faulted = v.status != 0;
+        res.peak_energy = v.peak;
+        // the step from buffer c ends in (c + 2) % 3
+        prev = a ? (cur + 2 * (a - 1)) % 3 : cur;
+        cur = (cur + 2 * a) % 3;
+    }
+    void run(int first, int count_steps, double dt, double frequency, double scale, double peak0, double *diag_out,
+             double *samples_out, swl_run2d &res) {
+        DeviceScope scope(device);
+        require(count_steps >= 0, "negative step count");
+        res.accepted = 0, res.faulted = 0, res.peak_energy = peak0;
+        if (!count_steps) return;
+        reserve_rows(count_steps);
+        run_begin(stream, peak0);
+        run_steps(stream, first, 0, count_steps, dt, frequency, scale);
+        Verdict v{};
+        run_end(stream, &v);
+        check(hipStreamSynchronize(stream), "2D run");
+        run_finish(v, diag_out, samples_out, res);
     }
 };
 

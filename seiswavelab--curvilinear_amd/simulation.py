@@ -393,7 +393,7 @@ class Batch2D:
     stops at its own total, and a member's fault latches on that member only (``Simulation.fault``)."""
 
     STAT_NAMES = ["grid", "captures", "replays", "direct_chunks", "chunks", "tune_ms_0", "tune_ms_1", "tune_ms_2",
-                  "tune_ms_3", "chunk", "capture_failures"]
+                  "tune_ms_3", "chunk", "capture_failures", "fused", "lanes", "lane_runs"]
 
     def __init__(self, device: int = 0):
         out = C.c_void_p()

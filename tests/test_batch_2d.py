@@ -2,8 +2,9 @@
 
 A shot stepped inside a batch runs the same stage arithmetic as alone, so its receivers and fields
 are bit-identical to the single-session run; only the energy sum is reduced in a different order
-(bar 1e-12 relative).  Covers mixed precision / GSLS mechanism counts / grid sizes, graph capture and
-replay, the persistent-grid autotuner, shots that finish at different steps, and fault isolation.
+(bar 1e-12 relative).  Covers both runner modes (per-shot kernels on several streams; one fused launch per stage over all
+shots), mixed precision / GSLS mechanism counts / grid sizes, graph capture and replay, the
+persistent-grid autotuner, shots that finish at different steps, and fault isolation.
 """
 import copy
 
@@ -53,14 +54,19 @@ def same(a, b):
     assert abs(da["energy"] - db["energy"]) <= 1e-12 * abs(da["energy"])
 
 
-@pytest.mark.parametrize("graphs", [1, 0])
-def test_batch_equals_alone(graphs):
+MODES = {"lanes": {"fused": 0, "lanes": 3, "lane_steps": 5},
+         "fused_graphs": {"fused": 1, "graphs": 1, "chunk": 16},
+         "fused_direct": {"fused": 1, "graphs": 0, "chunk": 16}}
+
+
+@pytest.mark.parametrize("mode", list(MODES))
+def test_batch_equals_alone(mode):
     ps = shots()
-    print(f"[batch] graphs={graphs}", flush=True)
+    print(f"[batch] mode={mode}", flush=True)
     ref = [alone(copy.deepcopy(p)) for p in ps]
     batch = Batch2D(0)
-    batch.set_option("graphs", graphs)
-    batch.set_option("chunk", 16)
+    for k, v in MODES[mode].items():
+        batch.set_option(k, v)
     members = [session(copy.deepcopy(p)) for p in ps]
     for s in members:
         batch.add(s)
@@ -71,17 +77,38 @@ def test_batch_equals_alone(graphs):
         assert s.completed()
         same(s, r)
     st = batch.stats()
-    assert st["chunks"] >= 5
-    if graphs:
-        assert st["captures"] >= 1 and st["replays"] >= 1
+    if mode == "lanes":
+        assert st["fused"] == 0 and st["lanes"] == 3 and st["lane_runs"] >= 1
     else:
+        assert st["chunks"] >= 5
+    if mode == "fused_graphs":
+        assert st["captures"] >= 1 and st["replays"] >= 1
+    elif mode == "fused_direct":
         assert st["direct_chunks"] == st["chunks"]
+
+
+def test_lanes_uneven_calls_equal_alone():
+    """Lanes with members finishing at different steps and calls that split runs unevenly."""
+    ps = shots()
+    ref = [alone(copy.deepcopy(p)) for p in ps]
+    batch = Batch2D(0)
+    batch.set_option("lanes", 2)
+    batch.set_option("lane_steps", 3)
+    members = [session(copy.deepcopy(p)) for p in ps]
+    for s in members:
+        batch.add(s)
+    for n in (1, 13, 29, 1000):
+        ok, err = batch.advance(n)
+        assert ok, err
+    for s, r in zip(members, ref):
+        same(s, r)
 
 
 def test_autotuner_choice_does_not_change_results():
     ps = shots()[:2]
     ref = [alone(copy.deepcopy(p)) for p in ps]
     batch = Batch2D(0)
+    batch.set_option("fused", 1)
     batch.set_option("chunk", 8)
     members = [session(copy.deepcopy(p)) for p in ps]
     for s in members:
@@ -95,12 +122,14 @@ def test_autotuner_choice_does_not_change_results():
         same(s, r)
 
 
-def test_fault_stays_with_its_session():
+@pytest.mark.parametrize("fused", [0, 1])
+def test_fault_stays_with_its_session(fused):
     ps = shots()[:3]
     # density 1e-300 overflows the source term (BackendValidation.cpp:55-62): the first step is rejected
     bad = Parameters(nx=40, nz=40, sourceZ=5, receiverCount=4, density=1e-300, vp=3000, vs=1700, maxSteps=20)
     ref = [alone(copy.deepcopy(p)) for p in ps[1:]]
     batch = Batch2D(0)
+    batch.set_option("fused", fused)
     members = [session(bad)] + [session(copy.deepcopy(p)) for p in ps[1:]]
     for s in members:
         batch.add(s)
