@@ -3,12 +3,14 @@
 Tolerances: FP64 HIP vs FP64 oracle <= 1e-10 relative L2 on every receiver gather and every
 field (the kernels keep the reference's operation order, so agreement is normally at the
 rounding level); FP32 HIP vs FP64 oracle <= 1e-4 relative L2 on the whole seismogram gather.
-Receiver index maps, PML slot maps and trace sample counts are compared exactly.
+Receiver index maps, PML slot maps and trace sample counts are compared exactly.  The same bars
+hold against the reference build's committed outputs (tests/golden/reference_2d_*.npz).
 """
 import numpy as np
 import pytest
 
 from cpu_sessions import CpuSession2D, rel_l2
+from reference_fixtures import CASES_2D, load_fixture, record_2d
 from seiswavelab_curvilinear_amd import (Absorber, BackendKind, GslsSettings, Layer, Parameters, Precision,
                                          Simulation, SourceKind, SourceMechanism, native)
 from test_oracle_2d import REF_CASES, precision_case, ref_case
@@ -148,3 +150,38 @@ def test_device_query():
     assert native.device_count() >= 1
     info = native.device_query(0)
     assert info.available and info.total_bytes > 0 and b"gfx9" in info.arch
+
+
+@pytest.mark.parametrize("name", CASES_2D)
+@pytest.mark.parametrize("prec", [Precision.Float64, Precision.Float32])
+def test_hip_matches_reference_fixtures(name, prec):
+    """Against the reference build's own FP64 outputs: dt and step count exactly; FP64 traces and final
+    field statistics <= 1e-10, energy after every step <= 1e-12 (a parallel sum); FP32 whole gather
+    <= 1e-4 and energy <= 1e-4."""
+    want = load_fixture("2d", name)
+    p = ref_case(name)
+    p.precision = prec
+    print(f"[hip2d fixture] {name} {prec.name}", flush=True)
+    s = Simulation()
+    ok, err = s.initialize(p, BackendKind.Hip, 0)
+    assert ok, err
+    assert s.dt() == float(want["dt"]) and s.totalSteps() == int(want["total"])
+    assert s.recordDt() == float(want["record_dt"])
+    energy, peak, stats = record_2d(s, s.totalSteps(),
+                                    lambda x: (x.diagnostics()["energy"], x.diagnostics()["peakEnergy"]),
+                                    lambda x: [x.snapshot_field(f) for f in range(x.layout()["fields"])])
+    r = s.receiver_arrays()
+    assert r["vx"].shape == want["vx"].shape
+    got_gather = np.concatenate([r["vx"].ravel(), r["vz"].ravel()])
+    want_gather = np.concatenate([want["vx"].ravel(), want["vz"].ravel()])
+    if prec == Precision.Float64:
+        assert rel_l2(want_gather, got_gather) <= FP64_TOL
+        np.testing.assert_allclose(energy, want["energy"], rtol=1e-12, atol=0)
+        np.testing.assert_allclose(peak, want["peak_energy"], rtol=1e-12, atol=0)
+        for f in range(len(stats)):
+            np.testing.assert_allclose(stats[f], want["field_stats"][f], rtol=FP64_TOL,
+                                       atol=FP64_TOL * want["field_stats"][f][0])
+    else:
+        assert rel_l2(want_gather, got_gather) <= FP32_GATHER_TOL
+        np.testing.assert_allclose(energy, want["energy"], rtol=1e-4, atol=1e-4 * want["energy"].max())
+    s.close()

@@ -3,11 +3,13 @@
 FP64: the engine follows the reference's operation order, so fields and traces match the oracle to
 rounding (bar: 1e-10 relative L2; the energy sum is a parallel reduction, bar 1e-12).  FP32: state
 in FP32, surface closure / projection / energy in FP64; bar 1e-4 relative L2 over the whole gather.
+The same bars hold against the reference build's committed outputs (tests/golden/reference_3d_*.npz).
 """
 import numpy as np
 import pytest
 
 from cpu_sessions import CpuSession3D, rel_l2
+from reference_fixtures import load_fixture, record_2d
 from seiswavelab_curvilinear_amd import BackendKind, Simulation3D
 from test_oracle_3d import CASES, case
 
@@ -117,3 +119,25 @@ def test_nonfinite_model_rejects_step_and_latches_fault_3d():
     assert np.all(np.isfinite(s.field(2)))
     ok, err2 = s.advance(1)
     assert not ok and err2 == err and s.currentStep() == step
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_fp64_matches_reference_fixtures_3d(name):
+    """Against the reference build's own outputs: dt and step count exactly; traces and final field
+    statistics <= 1e-10; energy after every step <= 1e-12 (a parallel sum)."""
+    want = load_fixture("3d", name)
+    p = case(name)
+    s = _gpu(p, 0)
+    assert s.dt() == float(want["dt"]) and s.totalSteps() == int(want["total"])
+    energy, peak, stats = record_2d(s, p.maxSteps,
+                                    lambda x: (x.diagnostics()["elasticEnergy"], x.diagnostics()["peakEnergy"]),
+                                    lambda x: [x.field(f) for f in range(9)])
+    r = s.receiver_arrays()
+    got = np.concatenate([r["vx"].ravel(), r["vy"].ravel(), r["vz"].ravel()])
+    ref = np.concatenate([want["vx"].ravel(), want["vy"].ravel(), want["vz"].ravel()])
+    assert r["vx"].shape == want["vx"].shape
+    assert rel_l2(ref, got) <= 1e-10
+    np.testing.assert_allclose(energy, want["energy"], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(peak, want["peak_energy"], rtol=1e-12, atol=0)
+    for f in range(9):
+        np.testing.assert_allclose(stats[f], want["field_stats"][f], rtol=1e-10, atol=1e-10 * want["field_stats"][f][0])

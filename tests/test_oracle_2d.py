@@ -4,7 +4,8 @@
   dry_loess_traces.csv, produced by the reference on its own build), copied under
   tests/golden/;
 * the reference's known answers (src/two_d/Validation2D.cpp:44, :69-70);
-* the reference build itself (oracle/_ref, same inputs, bitwise) where available;
+* the reference build itself (oracle/_ref, same inputs, bitwise) where available, and its
+  committed outputs on four cases (tests/golden/reference_2d_*.npz) everywhere;
 * closed forms: scipy PCHIP, flat-metric identity, conservative moment pattern.
 """
 import ctypes as C
@@ -15,6 +16,7 @@ import numpy as np
 import pytest
 
 from cpu_sessions import CpuSession2D, load, rel_l2
+from reference_fixtures import FIXTURES_2D, load_fixture, run_oracle_2d
 from seiswavelab_curvilinear_amd.params import (Absorber, CParams2D, GslsSettings, Layer, Parameters, Precision,
                                                  SourceKind, SourceMechanism)
 
@@ -244,3 +246,15 @@ def test_pml_slot_map_matches_reference():
                 if not (3 <= i < nx - 3 and 3 <= k < nz - 3):
                     continue
                 assert ref.ref2d_pml_slot(nx, nz, wx, wz, q, axis) == orc.orc2d_pml_slot(nx, nz, wx, wz, q, axis)
+
+
+@pytest.mark.parametrize("name,prec", FIXTURES_2D)
+def test_oracle_matches_reference_fixtures(name, prec):
+    """Bitwise against the reference build's committed outputs (dt, step count, energy per step,
+    receiver traces, final field statistics)."""
+    p = ref_case(name)
+    p.precision = prec
+    got, want = run_oracle_2d("orc", p), load_fixture("2d", name, prec)
+    assert sorted(got) == sorted(want)
+    for k in want:
+        assert np.array_equal(got[k], want[k]), k

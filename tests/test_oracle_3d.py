@@ -1,12 +1,14 @@
 """3D oracle (oracle/oracle3d.c) pinned against the reference build, closed forms and invariants.
 
 The reference's 3D CPU path (src/three_d/CurvilinearSolver.cpp, PointKernels3D.hpp) is compiled
-from its own sources into oracle/_ref/libswlref.so; every comparison with it is bit for bit.
+from its own sources into oracle/_ref/libswlref.so; every comparison with it is bit for bit, live where
+the build is present and against its committed outputs (tests/golden/reference_3d_*.npz) everywhere.
 """
 import numpy as np
 import pytest
 
 from cpu_sessions import CpuSession3D, load
+from reference_fixtures import load_fixture, run_oracle_3d
 from seiswavelab_curvilinear_amd.params import (LayerDepthMode3D, MaterialLayer3D, ReceiverArrayKind3D,
                                                 SimulationParameters, SourceKind, SourceMechanism, SurfaceGrid3D)
 
@@ -169,3 +171,13 @@ def test_invalid_parameters_rejected_with_reference_messages():
     p.receiverArray.kind, p.receiverArray.countX = ReceiverArrayKind3D.XLine, 500
     with pytest.raises(ValueError, match="Receiver counts exceed"):
         CpuSession3D("orc", p)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_oracle3d_matches_reference_fixtures(name):
+    """Bitwise against the reference build's committed outputs (dt, step count, energy per step,
+    receiver traces, final field statistics)."""
+    got, want = run_oracle_3d("orc", case(name)), load_fixture("3d", name)
+    assert sorted(got) == sorted(want)
+    for k in want:
+        assert np.array_equal(got[k], want[k]), k
