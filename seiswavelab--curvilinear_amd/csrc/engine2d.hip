@@ -85,6 +85,7 @@ template <class R> struct StageArgs {
     size_t count;
     R forcing, w, wy; // wy: weight of this stage's k in Y = U^n + dt(k1 + 2k2 + 2k3 + k4)/6 (FP32 interior)
     int kind;         // 0 first stage, 1 middle stages, 2 final stage (+ inspection)
+    int ystage;       // FP32 interior, stage 2: Y = U^n + dt/6 k1 is rebuilt from the input and the base
     const int *status;
     Partial *partials; // border tiles first, then interior tiles
     const int *tiles;  // border tile ids (64x4 tiles, gntx per row)
@@ -463,6 +464,7 @@ template <class R> __device__ inline R mad(R a, R b, R c) {
     else return a * b + c;
 }
 
+template <class R, int MECH> constexpr bool kRebuildY = kFast<R> && MECH == 0;
 template <class R, int DX, int DZ, int MECH>
 __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Partial *sh) {
     using V = typename Lane<R>::V;
@@ -619,9 +621,21 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
             V vo, vacc;
             if constexpr (kFast<R>) {
                 // FP32: the accumulator holds Y = U^n + dt/6 k1 + dt/3 k2 + dt/3 k3, so the last stage
-                // reads Y and the stage input only (the compulsory 3S traffic), not U^n as well
+                // reads Y and the stage input only (the compulsory 3S traffic), not U^n as well.  Stage 1
+                // does not store Y1 = U^n + dt/6 k1: stage 2 rebuilds it from what it reads anyway,
+                // Y1 = U^n + (U2 - U^n) / 3 (U2 = U^n + dt/2 k1), which saves one write and one read of S.
+                // Elastic only: for GSLS the extra live values cost more occupancy than the bytes save.
                 const V vb = A.kind == 2 ? V{} : A.kind == 0 && f < 5 ? centre[f < 5 ? f : 0] : vload(A.base + o);
-                const V vy = A.kind == 0 ? vb : vload(A.acc + o);
+                V vy;
+                if (A.kind == 0) {
+                    vy = vb;
+                } else if (kRebuildY<R, MECH> && A.ystage) {
+                    const V vin = f < 5 ? centre[f < 5 ? f : 0] : vload(A.in + o);
+#pragma unroll
+                    for (int j = 0; j < NV; ++j) vy[j] = mad(vin[j] - vb[j], (R)(1.0 / 3.0), vb[j]);
+                } else {
+                    vy = vload(A.acc + o);
+                }
 #pragma unroll
                 for (int j = 0; j < NV; ++j) {
                     if (A.kind == 2) {
@@ -645,7 +659,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
                 }
             }
             vstore(A.out + o, vo, A.nt);
-            if (A.kind != 2) vstore(A.acc + o, vacc, A.nt);
+            if (A.kind == 1 || (A.kind == 0 && !kRebuildY<R, MECH>)) vstore(A.acc + o, vacc, A.nt);
         }
         if (A.kind == 2) {
 #pragma unroll

@@ -330,6 +330,7 @@ template <class R> struct Engine2D {
         a.w = weights[s];
         a.wy = (R)(s == 0 || s == 3 ? dt / 6 : dt / 3);
         a.kind = s == 0 ? 0 : s == 3 ? 2 : 1;
+        a.ystage = s == 1;
         a.status = status.as<int>();
         a.partials = partials.as<Partial>();
         a.tiles = generalTiles.as<int>(), a.ngeneral = nGeneral, a.gblocks = (nGeneral + 7) / 8 * 8, a.gntx = ntx;
