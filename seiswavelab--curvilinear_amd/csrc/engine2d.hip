@@ -56,6 +56,7 @@ template <class R> struct DMesh {
     int fields, mech, has_damp, has_grad, metric;
     R h, hinv, rates[3], dir0, dir2; // hinv = 1/h (FP64: only when h is a power of two, so exact), else 0
     const R *a, *b, *jac, *lam, *mu, *rho, *src, *damp, *grad0, *grad1, *nrm, *cx, *cz;
+    const R *cxf, *czf; // FP32: per PML position 1/kappa, d/kappa + alpha, d/kappa^2 (no divisions per node)
     const R *dP[3], *dS[3];
     const double *sa, *sb, *slam, *smu, *snrm;   // FP64 surface row k = 3, indexed by i
     const double *tz_, *tslope, *fade, *fadeder; // separable metric profiles (by i / by k)
@@ -87,9 +88,10 @@ template <class R> struct StageArgs {
     int kind;         // 0 first stage, 1 middle stages, 2 final stage (+ inspection)
     int ystage;       // FP32 interior, stage 2: Y = U^n + dt/6 k1 is rebuilt from the input and the base
     const int *status;
-    Partial *partials; // border tiles first, then interior tiles
-    const int *tiles;  // border tile ids (64x4 tiles, gntx per row)
-    int ngeneral, gblocks, gntx, ninterior, rtx;
+    Partial *partials;       // border blocks first, then interior tiles
+    const int2 *border;      // (i, k) of every node outside the interior rectangle, row by row, 256 per block
+    int nborder;             // border nodes
+    int ngeneral, gblocks, ninterior, rtx;
     int nt; // interior: nontemporal stores
 };
 
@@ -97,6 +99,15 @@ template <class R> struct StageArgs {
 __device__ inline int tile_of_block(int b, int n) {
     const int x = b & 7, j = b >> 3, q = n >> 3, r = n & 7;
     return x * q + min(x, r) + j;
+}
+
+// FP64 keeps the reference's operation order exactly (separate multiply and add, true divisions), so
+// FP64 runs reproduce the reference to the bit.  FP32 is held to a tolerance, not to the bit, so it fuses
+// multiply-adds and replaces repeated divisions by reciprocals.
+template <class R> constexpr bool kFast = sizeof(R) == 4;
+template <class R> __device__ inline R mad(R a, R b, R c) {
+    if constexpr (kFast<R>) return __builtin_fmaf(a, b, c);
+    else return a * b + c;
 }
 
 // ---- MacCormack-DRP operators (FiniteDifference.hpp:14-25), direction as a template parameter
@@ -185,6 +196,12 @@ template <class R>
 __device__ inline R stretched(const DMesh<R> &m, const R *__restrict__ pin, long long slot, int at_, int axis, int field,
                               R d, R *rate) {
     if (slot < 0) return d;
+    if constexpr (kFast<R>) { // the same ADE with the profile's quotients precomputed per position
+        const R *pf = (axis ? m.czf : m.cxf) + 3 * at_;
+        const R psi = pin[slot + field];
+        rate[4 * axis + field] = -mad(pf[1], psi, pf[2] * d);
+        return mad(d, pf[0], psi);
+    }
     const R *pr = (axis ? m.cz : m.cx) + 3 * at_;
     const R dd = pr[0], kappa = pr[1], alpha = pr[2], psi = pin[slot + field];
     rate[4 * axis + field] = -(dd / kappa + alpha) * psi - dd / (kappa * kappa) * d;
@@ -257,7 +274,12 @@ __device__ inline void rhs_general(const DMesh<R> &m, const R *__restrict__ w, c
         dz[f] = b * uz;
     }
     const bool image = depth <= (DZ ? 3 : 1);
-    R o0 = (dx[2] + dz[4]) / rho, o1 = (dx[4] + dz[3]) / rho;
+    const R rinv = kFast<R> ? (R)1 / rho : (R)0; // FP32: one reciprocal instead of a division per term
+    auto per_rho = [&](R x) -> R {
+        if constexpr (kFast<R>) return x * rinv;
+        else return x / rho;
+    };
+    R o0 = per_rho(dx[2] + dz[4]), o1 = per_rho(dx[4] + dz[3]);
     if (image && !m.cpml) {
         o0 = momentum<R, DX, DZ>(m, w, count, q, 0, depth, -1);
         o1 = momentum<R, DX, DZ>(m, w, count, q, 1, depth, -1);
@@ -266,7 +288,7 @@ __device__ inline void rhs_general(const DMesh<R> &m, const R *__restrict__ w, c
 #pragma unroll
         for (int v = 0; v < 2; ++v) {
             const int f = v ? 4 : 2, other = v ? 3 : 4;
-            R x = xi[f] / rho, z = (a * eta[f] + b * eta[other]) / rho;
+            R x = per_rho(xi[f]), z = per_rho(a * eta[f] + b * eta[other]);
             if (image) {
                 x = momentum<R, DX, DZ>(m, w, count, q, v, depth, 0);
                 z = momentum<R, DX, DZ>(m, w, count, q, v, depth, 1);
@@ -277,12 +299,12 @@ __device__ inline void rhs_general(const DMesh<R> &m, const R *__restrict__ w, c
         }
     }
     if (m.has_grad) {
-        o0 += forcing * m.grad0[q] / rho;
-        o1 += forcing * m.grad1[q] / rho;
+        o0 += per_rho(forcing * m.grad0[q]);
+        o1 += per_rho(forcing * m.grad1[q]);
     } else {
         const R sw = m.src[q];
-        o0 += forcing * (sw * m.dir0) / rho;
-        o1 += forcing * (sw * m.dir2) / rho;
+        o0 += per_rho(forcing * (sw * m.dir0));
+        o1 += per_rho(forcing * (sw * m.dir2));
     }
     const R lambda = m.lam[q], mu = m.mu[q], div = dx[0] + dz[1];
     kout[0] = o0;
@@ -334,7 +356,8 @@ template <class R> __device__ inline R combine(int kind, R base, R k, R w, R &ac
         acc = acc + 2 * k;
         return base + w * k;
     }
-    return base + w * (acc + k) / (R)6;
+    if constexpr (kFast<R>) return mad(w * (acc + k), (R)(1.0 / 6.0), base);
+    else return base + w * (acc + k) / (R)6;
 }
 
 __device__ inline void merge(Partial &a, const Partial &b) {
@@ -379,14 +402,16 @@ __device__ inline void node_stats(R rho, R lam, R mu, R jac, R h, const R *u, bo
 }
 
 // ------------------------------------------------------------------------------------------------
-// General kernel: 64x4 tiles from a list; nodes inside the interior rectangle are skipped.
+// General kernel: block li takes border nodes [256 li, 256 li + 256) of the compacted list (every lane
+// holds a node; the lists follow the rows of 64x4 tiles, so a wave still reads row segments).
 template <class R, int DX, int DZ, int MECH>
 __device__ __forceinline__ void general_body(const StageArgs<R> &A, int li, Partial *sh) {
     constexpr int F = MECH ? 6 + 4 * MECH : 5;
-    const int t = A.tiles[li];
-    const int i = 3 + (t % A.gntx) * TX + (threadIdx.x % TX), k = 3 + (t / A.gntx) * TZ + (threadIdx.x / TX);
+    const int n = li * THREADS + (int)threadIdx.x;
+    const bool live = n < A.nborder;
+    const int2 ik = live ? A.border[n] : int2{3, 3};
+    const int i = ik.x, k = ik.y;
     const DMesh<R> &m = A.m;
-    const bool live = i < m.nx - 3 && k < m.nz - 3 && !in_rect(m, i, k);
     Partial p{0, 0, 0, 0, 0};
     if (live) {
         R kv[F], pr[8];
@@ -458,11 +483,6 @@ template <class R> __device__ inline void vstore(R *p, typename Lane<R>::V v, in
 // Interior arithmetic.  FP64 keeps the reference's operation order exactly (separate multiply and add,
 // true divisions), so FP64 runs reproduce the reference to the bit.  FP32 is held to a tolerance, not
 // to the bit, so its interior fuses multiply-adds and replaces repeated divisions by one reciprocal.
-template <class R> constexpr bool kFast = sizeof(R) == 4;
-template <class R> __device__ inline R mad(R a, R b, R c) {
-    if constexpr (kFast<R>) return __builtin_fmaf(a, b, c);
-    else return a * b + c;
-}
 
 template <class R, int MECH> constexpr bool kRebuildY = kFast<R> && MECH == 0;
 template <class R, int DX, int DZ, int MECH>

@@ -21,9 +21,9 @@ template <class R> struct Engine2D {
     DMesh<R> m{};
     DeviceBuffer mesh_arrays[32];
     DeviceBuffer state[3], accbuf, pstate[3], paccbuf, partials, level2, probes, status, accepted, peak, diag, samples,
-        gather, display, generalTiles;
+        gather, display, borderNodes;
     int ntx = 0, ntiles = 0;      // 64x4 tiles over the computational grid
-    int nGeneral = 0;             // tiles with at least one node outside the interior rectangle
+    int nBorder = 0, nGeneral = 0; // nodes outside the interior rectangle, blocks of 256 of them
     int rtx = 0, nInterior = 0;   // interior tiles (64*NV x 4 nodes)
     int nPartials = 0, nLevel2 = 0;
     hipEvent_t fork[4]{}, join[4]{};
@@ -125,6 +125,18 @@ template <class R> struct Engine2D {
         if (cpml) {
             m.cx = put_plain<R>(11, d.cpml_x, 3 * (size_t)(nx - 6));
             m.cz = put_plain<R>(12, d.cpml_z, 3 * (size_t)(nz - 6));
+            if (kFast<R>) { // 1/kappa, d/kappa + alpha, d/kappa^2 per position, evaluated in FP64
+                for (int axis = 0; axis < 2; ++axis) {
+                    const double *pr = axis ? d.cpml_z : d.cpml_x;
+                    const size_t n = (size_t)(axis ? nz : nx) - 6;
+                    std::vector<double> f(3 * n);
+                    for (size_t j = 0; j < n; ++j) {
+                        const double dd = pr[3 * j], kappa = pr[3 * j + 1], alpha = pr[3 * j + 2];
+                        f[3 * j] = 1 / kappa, f[3 * j + 1] = dd / kappa + alpha, f[3 * j + 2] = dd / (kappa * kappa);
+                    }
+                    (axis ? m.czf : m.cxf) = put_plain<R>(axis ? 29 : 28, f.data(), f.size());
+                }
+            }
         }
         for (int l = 0; l < mech; ++l)
             m.dP[l] = put_layout<R>(13 + l, d.delta_p[l], 1), m.dS[l] = put_layout<R>(16 + l, d.delta_s[l], 1);
@@ -263,18 +275,22 @@ template <class R> struct Engine2D {
                         m.si0 = std::min(m.si0, i), m.si1 = std::max(m.si1, i), m.sk0 = std::min(m.sk0, k),
                         m.sk1 = std::max(m.sk1, k);
         }
-        // general tiles: 64x4 tiles with at least one node outside the rectangle
+        // border nodes (outside the rectangle), compacted: visited tile by tile over 64x4 tiles, row by row
+        // inside each, so consecutive lanes read consecutive nodes of a row
         ntx = (nxc + TX - 1) / TX;
         ntiles = ntx * ((nzc + TZ - 1) / TZ);
-        std::vector<int> general;
+        std::vector<int2> border;
         for (int t = 0; t < ntiles; ++t) {
             const int ti0 = 3 + (t % ntx) * TX, ti1 = std::min(nx - 3, ti0 + TX), tk0 = 3 + (t / ntx) * TZ,
                       tk1 = std::min(nz - 3, tk0 + TZ);
-            const bool inside = ti0 >= i0 && ti1 <= i1 && tk0 >= k0 && tk1 <= k1;
-            if (!inside) general.push_back(t);
+            if (ti0 >= i0 && ti1 <= i1 && tk0 >= k0 && tk1 <= k1) continue;
+            for (int k = tk0; k < tk1; ++k)
+                for (int i = ti0; i < ti1; ++i)
+                    if (!(i >= i0 && i < i1 && k >= k0 && k < k1)) border.push_back(int2{i, k});
         }
-        nGeneral = (int)general.size();
-        generalTiles.upload(general.data(), std::max<size_t>(general.size(), 1) * sizeof(int));
+        nBorder = (int)border.size();
+        nGeneral = (nBorder + THREADS - 1) / THREADS;
+        borderNodes.upload(border.data(), std::max<size_t>(border.size(), 1) * sizeof(int2));
         rtx = i1 > i0 ? (i1 - i0 + 64 * NV - 1) / (64 * NV) : 0;
         nInterior = rtx ? rtx * ((k1 - k0 + 3) / 4) : 0;
         nPartials = nGeneral + nInterior;
@@ -333,7 +349,7 @@ template <class R> struct Engine2D {
         a.ystage = s == 1;
         a.status = status.as<int>();
         a.partials = partials.as<Partial>();
-        a.tiles = generalTiles.as<int>(), a.ngeneral = nGeneral, a.gblocks = (nGeneral + 7) / 8 * 8, a.gntx = ntx;
+        a.border = borderNodes.as<int2>(), a.nborder = nBorder, a.ngeneral = nGeneral, a.gblocks = (nGeneral + 7) / 8 * 8;
         a.ninterior = nInterior, a.rtx = rtx, a.nt = nt;
         return a;
     }
@@ -445,7 +461,7 @@ template <class R> struct Engine2D {
         for (const auto &x : state) b += x.bytes;
         for (const auto &x : pstate) b += x.bytes;
         for (const DeviceBuffer *x : {&accbuf, &paccbuf, &partials, &level2, &probes, &status, &accepted, &peak, &diag,
-                                      &samples, &gather, &display, &generalTiles})
+                                      &samples, &gather, &display, &borderNodes})
             b += x->bytes;
         return b;
     }
