@@ -282,6 +282,35 @@ int swl_batch2d_stats(const swl_batch2d *b, double *out);
 /* "" when the session has no latched fault */
 const char *swl_sim2d_fault(const swl_sim2d *s);
 
+/* ---------------- Receiver records as SEG-Y (SeisWave::writeSegy / readSegy, include/SeisCommon/SegyIo.hpp:5-19,
+   src/common/SegyIo.cpp) ----------------
+   Byte-compatible with the reference: SEG-Y rev 2 (rev 1.0 for the compatible profiles), big endian, one
+   component per file, 3200-byte textual header (CP037 for the IBM profile), 400-byte binary header with the
+   exact IEEE64 interval, 240-byte trace headers, coordinates in millimetres (scalar -1000). */
+typedef struct swl_segy_trace {
+    double position[3], source[3]; /* metres; z positive down */
+    double start_time;             /* seconds, a whole number of milliseconds */
+    const double *samples;         /* `count` samples of the selected component */
+} swl_segy_trace;
+/* profile: 0 ExactIeee32, 1 ExactIeee64, 2 LegacyIbm32, 3 CompatibleIeee32 (SegyExportProfile);
+   gain: SegyExportOptions::sampleGain; sample_interval: 0 keeps dt, > 0 resamples linearly (seconds).
+   Written to `path`.partial and renamed, so a reader sees the old file or the complete new one. */
+int swl_segy_write(const char *path, const swl_segy_trace *traces, int ntraces, int64_t count, double dt,
+                   const char *component, const char *units, int profile, double gain, double sample_interval);
+/* readSegy: rev 0/1/2, formats 1 (IBM), 2, 3, 5, 6, 8, either byte order, with the reference's limits. */
+typedef struct swl_segy_records swl_segy_records;
+int swl_segy_read(const char *path, swl_segy_records **out);
+void swl_segy_records_destroy(swl_segy_records *r);
+int swl_segy_records_info(const swl_segy_records *r, int *ntraces, double *dt, char *component, char *units,
+                          char *description, int text_capacity);
+int swl_segy_records_trace(const swl_segy_records *r, int index, double *position, double *source,
+                           double *start_time, int64_t *count);
+int swl_segy_records_samples(const swl_segy_records *r, int index, double *out);
+/* A 2D session's receivers, component 0 Vx / 1 Vz, as src/two_d/Experiment2D.cpp:44-71 assembles them: trace
+   positions (x, 0, z), the source at the coordinates of its node, dt = recordDt. */
+int swl_sim2d_write_segy(const swl_sim2d *s, const char *path, int component, int profile, double gain,
+                         double sample_interval);
+
 /* ---------------- 3D engine (replaces CudaEngine3D, src/three_d/CudaEngine3D.hpp:7-24) ---------------- */
 /* FP64 host mesh in padded storage q = i + nx*(j + ny*k) (Point::Mesh, src/three_d/PointKernels3D.hpp:13-21).
    Material arrays are per node (the homogeneous case repeats the constants).  The optional

@@ -275,3 +275,52 @@ int ref3d_model(void *h, int which, double *out) {
     }
 }
 } // extern "C"
+
+// ---- SEG-Y: the reference's own writer and reader (include/SeisCommon/SegyIo.hpp:5-19) over plain arrays
+#include "SeisCommon/SegyIo.hpp"
+extern "C" {
+// traces: n positions / sources (3 each), start times, samples (n x count, row per trace)
+int ref_segy_write(const char *path, const double *pos, const double *src, const double *start, const double *samples,
+                   int n, int64_t count, double dt, const char *component, const char *units, int profile,
+                   double gain, double interval, char *error, int len) {
+    SeisWave::RecordDataView view;
+    view.dt = dt, view.units = units, view.components = {component};
+    for (int r = 0; r < n; ++r) {
+        SeisWave::TraceView t;
+        for (int c = 0; c < 3; ++c) t.position[c] = pos[3 * r + c], t.source[c] = src[3 * r + c];
+        t.startTime = start[r];
+        t.channels[0] = std::span<const double>(samples + (size_t)r * count, (size_t)count);
+        view.traces.push_back(t);
+    }
+    SeisWave::SegyExportOptions o;
+    o.profile = static_cast<SeisWave::SegyExportProfile>(profile);
+    o.sampleGain = gain, o.sampleIntervalSeconds = interval;
+    std::string e;
+    const bool ok = SeisWave::writeSegy(view, 0, path, e, o);
+    copyError(e, error, len);
+    return ok;
+}
+void *ref_segy_read(const char *path, char *error, int len) {
+    auto r = std::make_unique<SeisWave::ImportedRecords>();
+    std::string e;
+    if (!SeisWave::readSegy(path, *r, e)) {
+        copyError(e, error, len);
+        return nullptr;
+    }
+    return r.release();
+}
+void ref_segy_free(void *h) { delete static_cast<SeisWave::ImportedRecords *>(h); }
+int ref_segy_info(void *h, int *n, double *dt, char *component, char *units, char *description, int cap) {
+    const auto &r = *static_cast<SeisWave::ImportedRecords *>(h);
+    *n = (int)r.traces.size(), *dt = r.dt;
+    copyError(r.component, component, cap), copyError(r.units, units, cap), copyError(r.description, description, cap);
+    return 1;
+}
+int ref_segy_trace(void *h, int i, double *pos, double *src, double *start, int64_t *count, double *samples) {
+    const auto &t = static_cast<SeisWave::ImportedRecords *>(h)->traces[i];
+    for (int c = 0; c < 3; ++c) pos[c] = t.position[c], src[c] = t.source[c];
+    *start = t.startTime, *count = (int64_t)t.samples.size();
+    if (samples) std::copy(t.samples.begin(), t.samples.end(), samples);
+    return 1;
+}
+} // extern "C"

@@ -7,4 +7,5 @@ from .params import (Absorber, AbsorbingBoundarySettings3D, BackendKind, CfsPmlS
                      LayerDepthMode3D, MaterialLayer3D, Parameters, Precision, ReceiverArrayKind3D,
                      ReceiverArraySettings3D, SimulationParameters, SourceKind, SourceMechanism, SurfaceGrid3D)
 from .simulation import Batch2D, Receiver, ReceiverTrace3D, Simulation, Simulation3D, writeReceivers
-from . import native, workloads
+from . import native, records, workloads
+from .records import ImportedRecords, SegyExportProfile, TraceRecord, read_segy, write_segy

@@ -73,6 +73,14 @@ SIGNATURES = {
     "swl_sim2d_read_frame": (C.c_int, [vp, C.c_int, vp, C.c_int]),
     "swl_sim2d_read_probes": (C.c_int, [vp, vp, C.c_int, vp]),
     "swl_sim2d_write_receivers": (C.c_int, [vp, C.c_char_p]),
+    "swl_segy_write": (C.c_int, [C.c_char_p, vp, C.c_int, C.c_int64, C.c_double, C.c_char_p, C.c_char_p, C.c_int,
+                                 C.c_double, C.c_double]),
+    "swl_segy_read": (C.c_int, [C.c_char_p, P(vp)]),
+    "swl_segy_records_destroy": (None, [vp]),
+    "swl_segy_records_info": (C.c_int, [vp, P(C.c_int), P(C.c_double), C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]),
+    "swl_segy_records_trace": (C.c_int, [vp, C.c_int, vp, vp, P(C.c_double), P(C.c_int64)]),
+    "swl_segy_records_samples": (C.c_int, [vp, C.c_int, vp]),
+    "swl_sim2d_write_segy": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_double]),
     "swl_pml_slot2d": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int]),
     "swl_params2d_size": (C.c_size_t, []),
     "swl_params3d_size": (C.c_size_t, []),

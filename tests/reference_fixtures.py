@@ -61,3 +61,32 @@ def run_oracle_3d(kind, p):
            "vx": r["vx"], "vy": r["vy"], "vz": r["vz"], "field_stats": stats}
     s.close()
     return out
+
+
+# ---- SEG-Y: one seeded record set written in five export variants by the reference's writeSegy
+SEGY_VARIANTS = {  # name: (profile, gain, sample interval s, source dt s, component, units)
+    "exact32": (0, 1.0, 0.0, 1.9310594e-4, "Vz", "m/s"),
+    "exact64": (1, 1.0, 0.0, 1.9310594e-4, "Vx", "m/s"),
+    "exact32_gain_resampled": (0, 2.5, 2.5e-4, 1.9310594e-4, "Vz", "m/s"),
+    "legacy_ibm": (2, 1.0, 2e-4, 1.9310594e-4, "Vx", "m/s"),
+    "compatible32": (3, 1.0, 0.0, 2.5e-4, "P", "Pa"),
+}
+
+
+def segy_records(seed=17, n=5, count=97):
+    """Positions, sources, start times and samples (n x count) of the synthetic record set."""
+    rng = np.random.default_rng(seed)
+    pos = np.column_stack([rng.uniform(0, 900, n), rng.uniform(-50, 50, n), rng.uniform(0, 12, n)])
+    pos[0, 1] = 0.0
+    src = np.tile([450.25, 0.0, 10.125], (n, 1))
+    start = np.zeros(n)
+    start[2] = 0.002
+    s = rng.standard_normal((n, count)) * 1e-6
+    s[0, :5] = 0.0
+    s[1, 3], s[1, 4], s[1, 5] = 1e-46, -3e-39, 7.5  # float32 underflow, IBM-representable small value, large
+    s[3, 10] = -0.5                                   # IBM fraction rounding at a hex-digit boundary
+    return pos, src, start, s
+
+
+def segy_fixture_path(name):
+    return GOLDEN / f"segy_{name}.sgy"

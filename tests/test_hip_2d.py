@@ -185,3 +185,35 @@ def test_hip_matches_reference_fixtures(name, prec):
         assert rel_l2(want_gather, got_gather) <= FP32_GATHER_TOL
         np.testing.assert_allclose(energy, want["energy"], rtol=1e-4, atol=1e-4 * want["energy"].max())
     s.close()
+
+
+def test_session_segy_export_round_trip(tmp_path):
+    """swl_sim2d_write_segy: the session's receivers as SEG-Y (Experiment2D.cpp:44-71 layout), read back
+    exactly (IEEE64 profile): samples, receiver positions (x, 0, z) to the millimetre, the source node's
+    coordinates, dt = recordDt."""
+    from seiswavelab_curvilinear_amd.records import SegyExportProfile, read_segy, write_session_segy
+    p = ref_case("moment_external_pml")
+    print("[hip2d segy] moment_external_pml", flush=True)
+    s = Simulation()
+    ok, err = s.initialize(p, BackendKind.Hip, 0)
+    assert ok, err
+    assert s.advance(s.totalSteps())[0]
+    rec = s.receiver_arrays()
+    L = s.layout()
+    coords = s.model_array("coordinates")
+    i = (p.sourceX if p.sourceX >= 0 else p.nx // 2) + 3 + L["offsetX"]
+    q = i + L["nx"] * (p.sourceZ + 3)
+    for comp, key in ((0, "vx"), (1, "vz")):
+        path = tmp_path / f"{key}.sgy"
+        ok, err = write_session_segy(s, path, comp, SegyExportProfile.ExactIeee64)
+        assert ok, err
+        ok, err, r = read_segy(path)
+        assert ok, err
+        assert r.dt == s.recordDt() * 1e6 * 1e-6 and r.component == ("Vx" if comp == 0 else "Vz")
+        assert len(r.traces) == rec[key].shape[0]
+        for n, t in enumerate(r.traces):
+            assert np.array_equal(t.samples, rec[key][n])
+            assert abs(t.position[0] - rec["position"][n][0]) <= 5e-4 and t.position[1] == 0
+            assert abs(t.position[2] - rec["position"][n][1]) <= 5e-4
+            assert abs(t.source[0] - coords[2 * q]) <= 5e-4 and abs(t.source[2] - coords[2 * q + 1]) <= 5e-4
+    s.close()
