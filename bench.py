@@ -69,6 +69,36 @@ def measure(pkg, params, steps, warmup, label, options=None):
     return {"wall": wall, "timing": timing, "nodes": nodes, "finite": d["finite"], "energy": d["energy"]}
 
 
+def accuracy(pkg, params, steps):
+    """FP32 against FP64 on the same model and steps: relative L2 and peak error of the whole receiver gather
+    (Vx and Vz of every receiver) and of the final wavefield snapshot (all five fields, each relative to its
+    own FP64 peak)."""
+    import copy
+    import numpy as np
+    runs = {}
+    for label, prec in (("f32", pkg.Precision.Float32), ("f64", pkg.Precision.Float64)):
+        p = copy.deepcopy(params)
+        p.precision = prec
+        s = pkg.Simulation()
+        ok, err = s.initialize(p, pkg.BackendKind.Hip, 0)
+        if not ok:
+            raise RuntimeError(err)
+        ok, err = s.advance(steps)
+        if not ok:
+            raise RuntimeError(err)
+        r = s.receiver_arrays()
+        runs[label] = (np.concatenate([r["vx"].ravel(), r["vz"].ravel()]), [s.snapshot_field(f) for f in range(5)])
+        s.close()
+    (g32, f32), (g64, f64) = runs["f32"], runs["f64"]
+    rel = lambda a, b: float(np.sqrt(np.sum((a - b) ** 2) / max(np.sum(b * b), 1e-300)))  # noqa: E731
+    peak = lambda a, b: float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))  # noqa: E731
+    out = {"steps": steps, "gather_rel_l2": rel(g32, g64), "gather_peak_error": peak(g32, g64),
+           "snapshot_rel_l2": max(rel(a, b) for a, b in zip(f32, f64)),
+           "snapshot_peak_error": max(peak(a, b) for a, b in zip(f32, f64))}
+    log(f"[bench] FP32 vs FP64 after {steps} steps: {out}")
+    return out
+
+
 def roofline(res, prec):
     t = res["timing"]
     avg_ms = t["stage_ms"] / max(t["stage_launches"], 1)
@@ -270,6 +300,8 @@ def main():
     ap.add_argument("--nz", type=int, default=2001)
     ap.add_argument("--nt", type=int, default=1, help="nontemporal stores in the interior kernel")
     ap.add_argument("--stored-metrics", type=int, default=0, help="read a, b, J from memory instead of recomputing")
+    ap.add_argument("--accuracy-steps", type=int, default=1000,
+                    help="steps of the FP32-vs-FP64 seismogram / snapshot comparison on the 5001x2001 shot (0 = skip)")
     ap.add_argument("--steps-3d", type=int, default=40, help="timed steps of the 400x288x200 3D shot (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=200, help="steps of each of the 64 GSLS shots (0 = skip)")
     ap.add_argument("--batch-shots", type=int, default=64)
@@ -310,6 +342,8 @@ def main():
         extra["fp64"] = {"value": round(r64["nodes"] * args.fp64_steps / r64["wall"] / 1e9, 4),
                          "ms_per_step": round(1e3 * r64["wall"] / args.fp64_steps, 4), "roofline": roof64}
         w.params.precision = pkg.Precision.Float32
+    if args.accuracy_steps > 0:
+        extra["fp32_vs_fp64"] = accuracy(pkg, w.params, args.accuracy_steps)
     if args.steps_3d > 0:
         extra["3d_elastic_400x288x200"] = {"fp64": measure_3d(pkg, args.steps_3d, 5, 0),
                                            "fp32": measure_3d(pkg, args.steps_3d, 5, 1)}
