@@ -18,7 +18,7 @@ ROOT = PKG.parent
 INCLUDE = ROOT / "include"
 LIB = CSRC / "libswl.so"
 
-HIP_SOURCES = ["engine2d.hip", "engine3d.hip"]
+HIP_SOURCES = ["engine2d.hip", "engine3d.hip", "geometry3d.hip"]
 CXX_SOURCES = ["model2d.cpp", "sim2d.cpp", "model3d.cpp", "sim3d.cpp", "batch2d.cpp", "segy.cpp"]
 
 

@@ -161,10 +161,28 @@ void validateGeometry(Model3D &m) {
         fail("Terrain transition must fit the model depth and exceed 2.6 times relief amplitude (at least four cells).");
 }
 
-// Grid3D::buildGeometry (CurvilinearSolver.cpp:146-190)
-void buildGeometry(Model3D &m) {
-    validateGeometry(m);
+// Grid3D::buildGeometry (CurvilinearSolver.cpp:146-190).  The surface displacement depends on (i, j) and the
+// fade on k only: both are evaluated once here (cos / pow on the host), the volume from them.
+void geometryInputs(const Model3D &m, std::vector<double> &surface, std::vector<double> &fade) {
     const auto &p = m.p;
+    const double dx = m.h;
+    surface.assign(static_cast<size_t>(m.nx) * m.ny, 0.0), fade.assign(m.nz, 0.0);
+    if (!m.terrain) return;
+    for (int j = 0; j < m.ny; ++j)
+        for (int i = 0; i < m.nx; ++i) {
+            const double x = (i - G) * dx, y = (j - G) * dx;
+            const double px = 2.0 * kPi * x / p.topography_wavelength, py = 2.0 * kPi * y / p.topography_wavelength;
+            surface[static_cast<size_t>(i) + static_cast<size_t>(m.nx) * j] =
+                hasDem(p) ? demDisplacement(p, x, y) : p.topography_amplitude * std::cos(px) * std::cos(py);
+        }
+    for (int k = 0; k < m.nz; ++k) {
+        const double depth = (k - G) * dx;
+        const double s = std::clamp(depth / m.transition, 0.0, 1.0);
+        fade[k] = depth >= m.transition ? 0.0
+                                        : 1.0 - std::pow(s, 5) * (126.0 + s * (-420.0 + s * (540.0 + s * (-315.0 + 70.0 * s))));
+    }
+}
+void buildGeometry(Model3D &m, const std::vector<double> &surface, const std::vector<double> &fade) {
     const double dx = m.h;
     m.coordinates.assign(3 * m.count, 0.0), m.metrics.assign(9 * m.count, 0.0);
     m.jacobian.assign(m.count, 0.0), m.normals.assign(3 * m.count, 0.0);
@@ -173,17 +191,8 @@ void buildGeometry(Model3D &m) {
             for (int i = 0; i < m.nx; ++i) {
                 const double x = (i - G) * dx, y = (j - G) * dx, depth = (k - G) * dx;
                 double *c = &m.coordinates[3 * m.index(i, j, k)];
-                if (!m.terrain) {
-                    c[0] = x, c[1] = y, c[2] = depth;
-                    continue;
-                }
-                const double px = 2.0 * kPi * x / p.topography_wavelength, py = 2.0 * kPi * y / p.topography_wavelength;
-                const double surface = hasDem(p) ? demDisplacement(p, x, y) : p.topography_amplitude * std::cos(px) * std::cos(py);
-                const double s = std::clamp(depth / m.transition, 0.0, 1.0);
-                const double fade = depth >= m.transition
-                                        ? 0.0
-                                        : 1.0 - std::pow(s, 5) * (126.0 + s * (-420.0 + s * (540.0 + s * (-315.0 + 70.0 * s))));
-                c[0] = x, c[1] = y, c[2] = depth + surface * fade;
+                c[0] = x, c[1] = y;
+                c[2] = m.terrain ? depth + surface[static_cast<size_t>(i) + static_cast<size_t>(m.nx) * j] * fade[k] : depth;
             }
     const auto coordDerivative = [&](int c, int i, int j, int k, int axis) {
         const int di = axis == 0, dj = axis == 1, dk = axis == 2;
@@ -234,27 +243,9 @@ int layerAt(const swl_params3d &p, double depth) {
     }
     return p.n_layers - 1;
 }
-} // namespace
-
-Model3D buildModel3D(const swl_params3d &requested) {
-    validate(requested);
-    Model3D m;
-    m.p = requested;
-    if (hasDem(requested)) {
-        m.dem.assign(requested.dem_elevations,
-                     requested.dem_elevations + static_cast<size_t>(requested.dem_nx) * requested.dem_ny);
-        m.p.dem_elevations = m.dem.data();
-    } else {
-        m.p.dem_elevations = nullptr, m.p.dem_nx = m.p.dem_ny = 0;
-    }
+// metric norm, Jacobian range, CFL plane distances, edges (CurvilinearSolver.cpp:328-362)
+void cflGeometry(Model3D &m) {
     const auto &p = m.p;
-    m.nx = p.nx + 2 * G, m.ny = p.ny + 2 * G, m.nz = p.nz + 2 * G;
-    m.count = static_cast<size_t>(m.nx) * m.ny * m.nz;
-    m.h = p.spacing;
-    m.transition = p.terrain_transition_depth;
-    m.terrain = hasDem(p) || p.topography_amplitude != 0;
-    buildGeometry(m);
-    // metric norm, Jacobian range, CFL plane distances, edges (CurvilinearSolver.cpp:328-362)
     m.minJacobian = std::numeric_limits<double>::max();
     double maxNorm = 0.0;
     for (int k = G; k < m.nz - G; ++k)
@@ -305,9 +296,39 @@ Model3D buildModel3D(const swl_params3d &requested) {
                     }
                 }
     m.maxEdge = maxEdge;
+}
+
+} // namespace
+
+Model3D buildModel3D(const swl_params3d &requested, int device) {
+    validate(requested);
+    Model3D m;
+    m.p = requested;
+    if (hasDem(requested)) {
+        m.dem.assign(requested.dem_elevations,
+                     requested.dem_elevations + static_cast<size_t>(requested.dem_nx) * requested.dem_ny);
+        m.p.dem_elevations = m.dem.data();
+    } else {
+        m.p.dem_elevations = nullptr, m.p.dem_nx = m.p.dem_ny = 0;
+    }
+    const auto &p = m.p;
+    m.nx = p.nx + 2 * G, m.ny = p.ny + 2 * G, m.nz = p.nz + 2 * G;
+    m.count = static_cast<size_t>(m.nx) * m.ny * m.nz;
+    m.h = p.spacing;
+    m.transition = p.terrain_transition_depth;
+    m.terrain = hasDem(p) || p.topography_amplitude != 0;
+    validateGeometry(m);
+    std::vector<double> surface, fade;
+    geometryInputs(m, surface, fade);
+    if (device >= 0) {
+        buildGeometry3DOnDevice(m, surface, fade, device); // the volume and the CFL geometry on the GPU (FP64)
+    } else {
+        buildGeometry(m, surface, fade);
+        cflGeometry(m);
+    }
     m.lambda = p.density * (p.vp * p.vp - 2.0 * p.vs * p.vs);
     m.mu = p.density * p.vs * p.vs;
-    if (!std::isfinite(m.lambda) || !std::isfinite(m.mu) || m.mu <= 0.0 || !std::isfinite(maxNorm))
+    if (!std::isfinite(m.lambda) || !std::isfinite(m.mu) || m.mu <= 0.0 || !std::isfinite(m.maxMetricNorm))
         fail("Derived elastic coefficients or grid metrics are out of range.");
     // materials (:366-381)
     double maxVp = p.vp, minVs = p.vs, maxVs = p.vs, maxDensity = p.density;
@@ -348,6 +369,7 @@ Model3D buildModel3D(const swl_params3d &requested) {
                 maxDamping = std::max(maxDamping, rate);
             }
     m.maxDamping = maxDamping;
+    const double maxNorm = m.maxMetricNorm, minLength = m.minLength;
     // dt plan (TimeStepPlan3D.hpp:8-20)
     for (double v : {p.spacing, p.cfl, p.duration, p.dominant_frequency, maxVp, maxNorm, minLength, maxDamping})
         if (!std::isfinite(v)) fail("Time-step inputs must be finite.");

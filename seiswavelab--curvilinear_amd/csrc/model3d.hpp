@@ -41,8 +41,12 @@ struct Model3D {
     size_t index(int i, int j, int k) const { return (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k); }
 };
 
-// Throws std::runtime_error with the reference's messages for invalid input.
-Model3D buildModel3D(const swl_params3d &requested);
+// Throws std::runtime_error with the reference's messages for invalid input.  device >= 0: the volume
+// geometry and the CFL reductions are computed on that GPU in FP64 (bit-identical to the host loops).
+Model3D buildModel3D(const swl_params3d &requested, int device = -1);
+// geometry3d.hip: coordinates, metrics, Jacobian, normals and the CFL figures from the per-column surface
+// displacement and the per-plane fade.
+void buildGeometry3DOnDevice(Model3D &m, const std::vector<double> &surface, const std::vector<double> &fade, int device);
 swl_mesh3d_desc meshDescriptor3D(const Model3D &m, int precision);
 
 } // namespace swl

@@ -88,7 +88,7 @@ int swl_sim3d_create(const swl_params3d *p, int backend, int device, int precisi
         swl::require(precision == 0 || precision == 1, "3D precision must be 0 (FP64) or 1 (FP32).");
         const auto begin = std::chrono::steady_clock::now();
         auto s = std::make_unique<swl_sim3d>();
-        s->model = swl::buildModel3D(*p);
+        s->model = swl::buildModel3D(*p, device); // FP64 volume geometry and CFL on the device
         s->device = device, s->precision = precision;
         const auto desc = swl::meshDescriptor3D(s->model, precision);
         if (swl_engine3d_create(&desc, s->model.receiverIndex.data(), static_cast<int>(s->model.receiverIndex.size()), device,

@@ -141,3 +141,45 @@ def test_fp64_matches_reference_fixtures_3d(name):
     np.testing.assert_allclose(peak, want["peak_energy"], rtol=1e-12, atol=0)
     for f in range(9):
         np.testing.assert_allclose(stats[f], want["field_stats"][f], rtol=1e-10, atol=1e-10 * want["field_stats"][f][0])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_device_geometry_equals_host_geometry_3d(name):
+    """A HIP session builds its coordinates, inverse metrics, Jacobian, normals and CFL figures on the GPU in FP64
+    (csrc/geometry3d.hip); they must equal the host builder's (= the oracle's, = the reference's) bit for bit, and
+    so must everything derived from them (materials, sponge, source, dt, step count)."""
+    from seiswavelab_curvilinear_amd import BackendKind as BK
+    p = case(name)
+    host = Simulation3D(BK.Cpu)
+    ok, err = host.initialize(p)
+    assert ok, err
+    dev = _gpu(p, 0)
+    assert dev.dt() == host.dt() and dev.totalSteps() == host.totalSteps()
+    for which in range(12):
+        assert np.array_equal(dev.model_array(which), host.model_array(which)), which
+    dh, dd = host.diagnostics(), dev.diagnostics()
+    for k in ("minJacobian", "maxMetricNorm", "minCellLength", "maxCellEdge", "waveDtLimit", "dampingDtLimit",
+              "sourceDtLimit"):
+        assert dh[k] == dd[k], k
+
+
+def test_device_geometry_dem_workload_3d():
+    """The 400x288x200 DEM workload: GPU-built geometry equals the host-built one (dt, CFL figures, metrics)."""
+    from seiswavelab_curvilinear_amd import BackendKind as BK
+    from seiswavelab_curvilinear_amd import workloads
+    import time
+    p, _ = workloads.dem_terrain_400x288x200(steps=4)
+    host = Simulation3D(BK.Cpu)
+    t0 = time.perf_counter()
+    ok, err = host.initialize(p)
+    assert ok, err
+    t_host = time.perf_counter() - t0
+    dev = _gpu(p, 0)
+    assert dev.dt() == host.dt() and dev.totalSteps() == host.totalSteps()
+    dh, dd = host.diagnostics(), dev.diagnostics()
+    for k in ("minJacobian", "maxMetricNorm", "minCellLength", "maxCellEdge", "waveDtLimit"):
+        assert dh[k] == dd[k], k
+    for which in (0, 1, 2, 3):
+        assert np.array_equal(dev.model_array(which), host.model_array(which)), which
+    print(f"[3d geometry] host plan {1e3 * t_host:.0f} ms, HIP session (geometry on the GPU, engine set-up, "
+          f"upload) {dd['initializationMs']:.0f} ms", flush=True)
