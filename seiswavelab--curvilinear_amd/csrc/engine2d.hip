@@ -485,23 +485,79 @@ template <class R> __device__ inline void vstore(R *p, typename Lane<R>::V v, in
 // to the bit, so its interior fuses multiply-adds and replaces repeated divisions by one reciprocal.
 
 template <class R, int MECH> constexpr bool kRebuildY = kFast<R> && MECH == 0;
-template <class R, int DX, int DZ, int MECH>
-__device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Partial *sh) {
+// LDS staging of the stencil input (the `tile` variant): per field, the block's 4 rows plus the 3 + 1
+// halo rows of the z-stencil (8 rows) and HV halo vectors on each side of the 64 lane vectors are loaded
+// once into LDS (double-buffered over the fields); the lanes then read their x- and z-taps from LDS.
+template <class R> struct Tile {
+    static constexpr int NV = Lane<R>::N, HV = (3 + NV - 1) / NV, W = 64 + 2 * HV, ROWS = 8;
+    using V = typename Lane<R>::V;
+    V v[2][ROWS][W];
+};
+template <class R, int DX, int DZ, int MECH, bool TILE = false>
+__device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Partial *sh, Tile<R> *tile = nullptr) {
     using V = typename Lane<R>::V;
     constexpr int NV = Lane<R>::N;
     constexpr int F = MECH ? 6 + 4 * MECH : 5;
     const DMesh<R> &m = A.m;
-    const int i0 = m.ri0 + ((tl % A.rtx) * 64 + (threadIdx.x % 64)) * NV, k = m.rk0 + (tl / A.rtx) * 4 + threadIdx.x / 64;
+    const int bx = m.ri0 + (tl % A.rtx) * 64 * NV, bk = m.rk0 + (tl / A.rtx) * 4; // block's first node, first row
+    const int lane = threadIdx.x % 64, wrow = threadIdx.x / 64;
+    const int i0 = bx + lane * NV, k = bk + wrow;
     const bool live = i0 < m.ri1 && k < m.rk1;
     Partial p{0, 0, 0, 0, 0};
+    R xi[5][NV], eta[5][NV];
+    V centre[5];
+    if constexpr (TILE) {
+        using T = Tile<R>;
+        constexpr int rlo = DZ ? -3 : -1; // first staged row relative to bk
+        const size_t cnt = A.count;
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+            auto &buf = tile->v[f & 1];
+            // rows bk + rlo .. bk + rlo + 7 of vectors bx - HV NV .. bx + (64 + HV) NV; nothing outside the
+            // allocation is read (rows < nz, vectors inside the row's ld span; the rest are only dead lanes')
+            for (int e = threadIdx.x; e < T::ROWS * T::W; e += THREADS) {
+                const int r = e / T::W, v = e % T::W, kr = bk + rlo + r, ix = bx + (v - T::HV) * NV;
+                V x{};
+                if (kr < m.nz && m.off + ix >= 0 && m.off + ix + NV <= m.ld) x = vload(A.in + (size_t)f * cnt + at(m, ix, kr));
+                buf[r][v] = x;
+            }
+            __syncthreads();
+            if (live) {
+                const int rr = wrow - rlo;
+                R row[(2 * T::HV + 1) * NV];
+#pragma unroll
+                for (int v = 0; v <= 2 * T::HV; ++v) {
+                    const V x = buf[rr][lane + v];
+                    if (v == T::HV) centre[f] = x;
+#pragma unroll
+                    for (int j = 0; j < NV; ++j) row[v * NV + j] = x[j];
+                }
+#pragma unroll
+                for (int j = 0; j < NV; ++j) {
+                    R r = 0;
+#pragma unroll
+                    for (int l = 0; l < 5; ++l) r = mad(coef<R, DX>(l), row[T::HV * NV + j + offs<DX>(l)], r);
+                    xi[f][j] = over_h(r, m.h, m.hinv);
+                }
+                V col[5];
+#pragma unroll
+                for (int l = 0; l < 5; ++l) col[l] = buf[rr + offs<DZ>(l)][lane + T::HV];
+#pragma unroll
+                for (int j = 0; j < NV; ++j) {
+                    R r = 0;
+#pragma unroll
+                    for (int l = 0; l < 5; ++l) r = mad(coef<R, DZ>(l), col[l][j], r);
+                    eta[f][j] = over_h(r, m.h, m.hinv);
+                }
+            }
+        }
+    }
     if (live) {
         const ptrdiff_t q = at(m, i0, k), s = m.ld;
         const size_t cnt = A.count;
         const R h = m.h;
-        R xi[5][NV], eta[5][NV];
-        V centre[5];
 #pragma unroll
-        for (int f = 0; f < 5; ++f) {
+        for (int f = 0; f < 5 && !TILE; ++f) {
             const R *fp = A.in + (size_t)f * cnt;
             // x: the lane's NV nodes plus HV vectors on each side (the biased stencil reaches 3 nodes)
             constexpr int HV = (3 + NV - 1) / NV;
@@ -705,7 +761,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
 // take the interior tiles.  A single in-order launch avoids cross-stream joins between stages.
 // The FP32 elastic stage fits 96 VGPRs without spilling, which gives 5 waves per SIMD instead of 4.
 template <class R, int MECH> constexpr int kWavesPerEU = sizeof(R) == 4 && MECH == 0 ? 5 : 1;
-template <class R, int DX, int DZ, int MECH>
+template <class R, int DX, int DZ, int MECH, bool TILE = false>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<R, MECH>))) void stage_kernel(
     StageArgs<R> A) {
     __shared__ Partial sh[THREADS];
@@ -713,6 +769,9 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(kWavesP
     const int b = blockIdx.x;
     if (b < A.gblocks) {
         if (b < A.ngeneral) general_body<R, DX, DZ, MECH>(A, b, sh);
+    } else if constexpr (TILE) {
+        __shared__ Tile<R> tile;
+        interior_body<R, DX, DZ, MECH, true>(A, tile_of_block(b - A.gblocks, A.ninterior), sh, &tile);
     } else {
         interior_body<R, DX, DZ, MECH>(A, tile_of_block(b - A.gblocks, A.ninterior), sh);
     }

@@ -28,6 +28,7 @@ template <class R> struct Engine2D {
     int nPartials = 0, nLevel2 = 0;
     hipEvent_t fork[4]{}, join[4]{};
     int cur = 0, prev = 0, diag_rows = 0, registers = 0, nt = 1, metricOk = 0;
+    int tileLds = 0; // elastic interior: stencil input staged through LDS tiles
     size_t local_bytes = 0;
     std::vector<Partial> host_partials;
     // optional per-launch timing with HIP events (interior stage kernel and step finalisation)
@@ -305,7 +306,10 @@ template <class R> struct Engine2D {
         const int blocks = a.gblocks + nInterior;
         if (slot == 0) mark(0, true);
         switch (mech) {
-        case 0: stage_kernel<R, DX, DZ, 0><<<blocks, THREADS, 0, stream>>>(a); break;
+        case 0:
+            if (tileLds) stage_kernel<R, DX, DZ, 0, true><<<blocks, THREADS, 0, stream>>>(a);
+            else stage_kernel<R, DX, DZ, 0><<<blocks, THREADS, 0, stream>>>(a);
+            break;
         case 1: stage_kernel<R, DX, DZ, 1><<<blocks, THREADS, 0, stream>>>(a); break;
         case 2: stage_kernel<R, DX, DZ, 2><<<blocks, THREADS, 0, stream>>>(a); break;
         default: stage_kernel<R, DX, DZ, 3><<<blocks, THREADS, 0, stream>>>(a); break;
@@ -640,6 +644,7 @@ int swl_engine2d_set_option(swl_engine2d *e, const char *name, int value) {
         e->with([&](auto &x) {
             if (key == "nontemporal_stores") x.nt = value != 0;
             else if (key == "stored_metrics") x.m.metric = value == 0 && x.metricOk; // read a, b, J from memory
+            else if (key == "lds_tiles") x.tileLds = value != 0;
             else throw std::runtime_error("unknown engine option: " + key);
         });
     });

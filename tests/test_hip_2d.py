@@ -217,3 +217,24 @@ def test_session_segy_export_round_trip(tmp_path):
             assert abs(t.position[2] - rec["position"][n][1]) <= 5e-4
             assert abs(t.source[0] - coords[2 * q]) <= 5e-4 and abs(t.source[2] - coords[2 * q + 1]) <= 5e-4
     s.close()
+
+
+@pytest.mark.parametrize("prec", [Precision.Float32, Precision.Float64])
+def test_lds_tiles_equal_direct_loads(prec):
+    """The LDS-staged interior (option "lds_tiles") reads the same values as the direct-load interior, so
+    traces and fields are bit-identical."""
+    p = ref_case("sine_terrain_small")
+    p.precision = prec
+    runs = []
+    for tiles in (0, 1):
+        s = Simulation()
+        ok, err = s.initialize(p, BackendKind.Hip, 0)
+        assert ok, err
+        s.set_option("lds_tiles", tiles)
+        assert s.advance(s.totalSteps())[0]
+        runs.append((s.receiver_arrays(), [s.snapshot_field(f) for f in range(5)]))
+        s.close()
+    (ra, fa), (rb, fb) = runs
+    assert np.array_equal(ra["vx"], rb["vx"]) and np.array_equal(ra["vz"], rb["vz"])
+    for f in range(5):
+        assert np.array_equal(fa[f], fb[f]), f
