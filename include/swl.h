@@ -282,6 +282,35 @@ int swl_batch2d_stats(const swl_batch2d *b, double *out);
 /* "" when the session has no latched fault */
 const char *swl_sim2d_fault(const swl_sim2d *s);
 
+/* ---------------- Background stepping (Seis2D::Curvilinear::AsyncSimulation, include/Seis2D/AsyncSimulation2D.hpp,
+   src/two_d/AsyncSimulation2D.cpp) ----------------
+   One persistent worker per object.  queue(steps, budget_ms) runs at most 8 single steps (stopping at the time
+   budget, at completion or on pause) and publishes step, diagnostics, the receiver samples recorded since the
+   batch was queued and, if requested, a display frame; poll / synchronize fold the publication into the state
+   that the accessors below return.  synchronize, advance, reset, initialize and session() cancel a running
+   batch between steps and wait for it. */
+typedef struct swl_async2d swl_async2d;
+int swl_async2d_create(swl_async2d **out);
+void swl_async2d_destroy(swl_async2d *a);
+int swl_async2d_initialize(swl_async2d *a, const swl_params2d *p, int backend, int device);
+/* returns 1 when a batch was queued, 0 when not initialized, completed or busy (AsyncSimulation::queue) */
+int swl_async2d_queue(swl_async2d *a, int steps, double budget_ms);
+int swl_async2d_poll(swl_async2d *a);          /* non-zero: the last batch's error (swl_last_error) */
+void swl_async2d_pause(swl_async2d *a);
+int swl_async2d_synchronize(swl_async2d *a);
+int swl_async2d_busy(swl_async2d *a);
+/* BackgroundTiming: compute ms, transfer ms, batches, steps */
+int swl_async2d_timing(const swl_async2d *a, double *out);
+int swl_async2d_advance(swl_async2d *a, int count);
+int swl_async2d_reset(swl_async2d *a);
+/* published state: initialized, completed, step, total steps, receivers, samples per receiver */
+int swl_async2d_state(const swl_async2d *a, int64_t *out);
+int swl_async2d_times(const swl_async2d *a, double *out); /* dt, recordDt */
+int swl_async2d_diagnostics(const swl_async2d *a, double *out); /* as swl_sim2d_diagnostics */
+int swl_async2d_receivers(const swl_async2d *a, double *vx, double *vz);
+int swl_async2d_read_frame(swl_async2d *a, int component, float *out, int include_absorber, int *ready);
+swl_sim2d *swl_async2d_session(swl_async2d *a);
+
 /* ---------------- Receiver records as SEG-Y (SeisWave::writeSegy / readSegy, include/SeisCommon/SegyIo.hpp:5-19,
    src/common/SegyIo.cpp) ----------------
    Byte-compatible with the reference: SEG-Y rev 2 (rev 1.0 for the compatible profiles), big endian, one

@@ -19,7 +19,7 @@ INCLUDE = ROOT / "include"
 LIB = CSRC / "libswl.so"
 
 HIP_SOURCES = ["engine2d.hip", "engine3d.hip", "geometry3d.hip"]
-CXX_SOURCES = ["model2d.cpp", "sim2d.cpp", "model3d.cpp", "sim3d.cpp", "batch2d.cpp", "segy.cpp"]
+CXX_SOURCES = ["model2d.cpp", "sim2d.cpp", "model3d.cpp", "sim3d.cpp", "batch2d.cpp", "segy.cpp", "async2d.cpp"]
 
 
 def hipcc() -> str:

@@ -81,6 +81,23 @@ SIGNATURES = {
     "swl_segy_records_trace": (C.c_int, [vp, C.c_int, vp, vp, P(C.c_double), P(C.c_int64)]),
     "swl_segy_records_samples": (C.c_int, [vp, C.c_int, vp]),
     "swl_sim2d_write_segy": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_double]),
+    "swl_async2d_create": (C.c_int, [P(vp)]),
+    "swl_async2d_destroy": (None, [vp]),
+    "swl_async2d_initialize": (C.c_int, [vp, vp, C.c_int, C.c_int]),
+    "swl_async2d_queue": (C.c_int, [vp, C.c_int, C.c_double]),
+    "swl_async2d_poll": (C.c_int, [vp]),
+    "swl_async2d_pause": (None, [vp]),
+    "swl_async2d_synchronize": (C.c_int, [vp]),
+    "swl_async2d_busy": (C.c_int, [vp]),
+    "swl_async2d_timing": (C.c_int, [vp, vp]),
+    "swl_async2d_advance": (C.c_int, [vp, C.c_int]),
+    "swl_async2d_reset": (C.c_int, [vp]),
+    "swl_async2d_state": (C.c_int, [vp, vp]),
+    "swl_async2d_times": (C.c_int, [vp, vp]),
+    "swl_async2d_diagnostics": (C.c_int, [vp, vp]),
+    "swl_async2d_receivers": (C.c_int, [vp, vp, vp]),
+    "swl_async2d_read_frame": (C.c_int, [vp, C.c_int, vp, C.c_int, P(C.c_int)]),
+    "swl_async2d_session": (vp, [vp]),
     "swl_pml_slot2d": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int]),
     "swl_params2d_size": (C.c_size_t, []),
     "swl_params3d_size": (C.c_size_t, []),

@@ -226,6 +226,148 @@ class Simulation:
         return True, "", out
 
 
+class _SessionView(Simulation):
+    """A Simulation over a session owned by someone else (an AsyncSimulation): never destroys it."""
+
+    def __init__(self, handle, params):
+        super().__init__()
+        self._h, self._params = handle, params
+
+    def close(self):
+        self._h = None
+
+    def __del__(self):
+        self._h = None
+
+
+class AsyncSimulation:
+    """Background stepping of a 2D session: the reference's AsyncSimulation (AsyncSimulation2D.hpp:5-22) over
+    swl_async2d_*.  ``queue(steps, budgetMs)`` runs at most 8 steps on a worker thread; ``poll`` / ``synchronize``
+    fold the published batch (step, diagnostics, receiver samples, a requested display frame) into the state
+    the accessors return; ``pause`` stops a batch between steps."""
+
+    def __init__(self):
+        out = C.c_void_p()
+        native.call("swl_async2d_create", C.byref(out))
+        self._h = out
+        self._params = None
+
+    def close(self):
+        if self._h:
+            native.lib().swl_async2d_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _state(self):
+        o = np.zeros(6, np.int64)
+        native.lib().swl_async2d_state(self._h, _dp(o))
+        return o
+
+    def initialize(self, params: Parameters, backend: BackendKind = BackendKind.Hip, device: int = 0):
+        c, keep = to_c(params)
+        rc = native.lib().swl_async2d_initialize(self._h, C.byref(c), int(backend), int(device))
+        if rc != 0:
+            return False, native.last_error()
+        self._params = params
+        self._layout = self.session().layout()  # fixed for the session's lifetime
+        return True, ""
+
+    def queue(self, steps: int, budgetMs: float) -> bool:
+        return native.lib().swl_async2d_queue(self._h, int(steps), float(budgetMs)) == 1
+
+    def poll(self):
+        rc = native.lib().swl_async2d_poll(self._h)
+        return (rc == 0), ("" if rc == 0 else native.last_error())
+
+    def pause(self):
+        native.lib().swl_async2d_pause(self._h)
+
+    def synchronize(self):
+        if native.lib().swl_async2d_synchronize(self._h) != 0:
+            raise RuntimeError(native.last_error())
+
+    def busy(self) -> bool:
+        return native.lib().swl_async2d_busy(self._h) == 1
+
+    def backgroundTiming(self) -> dict:
+        o = np.zeros(4)
+        native.lib().swl_async2d_timing(self._h, _dp(o))
+        return {"computeMs": o[0], "transferMs": o[1], "batches": int(o[2]), "steps": int(o[3])}
+
+    def advance(self, count: int):
+        rc = native.lib().swl_async2d_advance(self._h, int(count))
+        return (rc == 0), ("" if rc == 0 else native.last_error())
+
+    def reset(self):
+        rc = native.lib().swl_async2d_reset(self._h)
+        return (rc == 0), ("" if rc == 0 else native.last_error())
+
+    def initialized(self) -> bool:
+        return bool(self._state()[0])
+
+    def completed(self) -> bool:
+        return bool(self._state()[1])
+
+    def step(self) -> int:
+        return int(self._state()[2])
+
+    def totalSteps(self) -> int:
+        return int(self._state()[3])
+
+    def dt(self) -> float:
+        o = np.zeros(2)
+        native.lib().swl_async2d_times(self._h, _dp(o))
+        return float(o[0])
+
+    def recordDt(self) -> float:
+        o = np.zeros(2)
+        native.lib().swl_async2d_times(self._h, _dp(o))
+        return float(o[1])
+
+    def parameters(self) -> Parameters:
+        return self._params
+
+    def diagnostics(self) -> dict:
+        o = np.zeros(len(DIAGNOSTIC_NAMES))
+        native.lib().swl_async2d_diagnostics(self._h, _dp(o))
+        d = dict(zip(DIAGNOSTIC_NAMES, o.tolist()))
+        d["finite"] = bool(d["finite"])
+        return d
+
+    def receiver_arrays(self):
+        s = self._state()
+        nr, n = int(s[4]), int(s[5])
+        vx, vz = np.zeros((nr, n)), np.zeros((nr, n))
+        native.lib().swl_async2d_receivers(self._h, _dp(vx), _dp(vz))
+        return {"vx": vx, "vz": vz}
+
+    def readFrame(self, component: int, includeAbsorber: bool = False):
+        """(ok, error, frame).  While a batch runs and the current step's frame is not yet published it returns
+        (False, "", None) and remembers the request for the next batch, as the reference does."""
+        L, p = self._layout, self._params
+        nx = L["nx"] - 6 if includeAbsorber else p.nx
+        nz = L["nz"] - 6 if includeAbsorber else p.nz
+        out = np.zeros(nx * nz, np.float32)
+        ready = C.c_int()
+        rc = native.lib().swl_async2d_read_frame(self._h, int(component), _dp(out), int(includeAbsorber),
+                                                  C.byref(ready))
+        if rc != 0:
+            return False, native.last_error(), None
+        return (ready.value == 1), "", (out.reshape(nz, nx) if ready.value else None)
+
+    def session(self) -> Simulation:
+        """The session itself after the running batch has finished (operator const Simulation&)."""
+        return _SessionView(native.lib().swl_async2d_session(self._h), self._params)
+
+    def snapshot(self, includeMemories: bool = False) -> dict:
+        return self.session().snapshot(includeMemories)
+
+
 def writeReceivers(sim: Simulation, path: str):
     """Receiver CSV with the reference's provenance header (CurvilinearSolver2D.cpp:811-854)."""
     rc = native.lib().swl_sim2d_write_receivers(sim._h, str(path).encode())
