@@ -62,6 +62,9 @@ typedef struct swl_params2d {
     const double *surface_xz;
     const double *node_vp, *node_vs, *node_density;
     int threads;
+    /* receiver policy of the interactive survey (Parameters::autoSurfaceReceivers / receiverTargetCount,
+       CurvilinearSolver2D.hpp:45-46): stored and saved with the model; automatic layout is not provided */
+    int auto_surface_receivers, receiver_target_count;
 } swl_params2d;
 
 /* Mirrors Seis3D::Curvilinear::SimulationParameters (include/Seis3D/CurvilinearSolver.hpp:70-95). */
@@ -339,6 +342,29 @@ int swl_segy_records_samples(const swl_segy_records *r, int index, double *out);
    positions (x, 0, z), the source at the coordinates of its node, dt = recordDt. */
 int swl_sim2d_write_segy(const swl_sim2d *s, const char *path, int component, int profile, double gain,
                          double sample_interval);
+
+/* ---------------- 2D model documents and wavefield snapshots as ASCII VTS ----------------
+   Model documents are byte-compatible with the reference's (versioned FieldData parameter record, six point
+   arrays, coordinates, FNV-1a 64-bit checksum, include/SeisCommon/VtsModelIo.hpp:18-86).  A document is a
+   model, not a wavefield checkpoint: a session built from it starts at t = 0. */
+/* writeModelVts (include/Seis2D/ModelIo2D.hpp:8-10, src/two_d/ModelIo2D.cpp:75-144).  cpu_threads and
+   cuda_device are the ComputeSettings3D recorded in the document; BackendKind is 1 for a HIP session (the
+   accelerator slot of BackendKind3D) and 0 for a device-free plan.  Written to `path`.tmp-<ticks> and renamed,
+   so an existing model survives a failed write. */
+int swl_sim2d_write_model_vts(const swl_sim2d *s, const char *path, int cpu_threads, int cuda_device);
+/* readModelVts (ModelIo2D.hpp:11-13, ModelIo2D.cpp:146-254): validates shape, finite data, schema 1..7 and
+   the checksum; third-party VTS and wavefield snapshots are rejected.  On failure *out is untouched. */
+typedef struct swl_model_doc2d swl_model_doc2d;
+int swl_model_vts_read(const char *path, swl_model_doc2d **out);
+void swl_model_doc2d_destroy(swl_model_doc2d *d);
+/* ModelDocument::parameters; surface_xz points into the document (valid until it is destroyed). */
+int swl_model_doc2d_params(const swl_model_doc2d *d, swl_params2d *out);
+/* out[5]: backend kind, compute.cpuThreads, compute.cudaDevice, referenceDt, schema version. */
+int swl_model_doc2d_info(const swl_model_doc2d *d, double *out);
+/* writeSnapshot (CurvilinearSolver2D.hpp:124-139, CurvilinearSolver2D.cpp:725-810).  components[7]: Vx, Vz,
+   velocity magnitude, pressure, Sxx, Szz, Sxz (SnapshotExportOptions::components); include_material adds Vp,
+   Vs, Density (and Qp, Qs when attenuating).  An empty selection fails before the file is touched. */
+int swl_sim2d_write_snapshot(const swl_sim2d *s, const char *path, const int32_t *components, int include_material);
 
 /* ---------------- 3D engine (replaces CudaEngine3D, src/three_d/CudaEngine3D.hpp:7-24) ---------------- */
 /* FP64 host mesh in padded storage q = i + nx*(j + ny*k) (Point::Mesh, src/three_d/PointKernels3D.hpp:13-21).

@@ -56,6 +56,7 @@ typedef struct orc_params2d {
        and ghosts take the nearest logical node.  Not expressible in the reference. */
     const double *node_vp, *node_vs, *node_density;
     int threads;                           /* CPU workers; 0 = min(8, procs) */
+    int auto_surface_receivers, receiver_target_count; /* receiver policy (stored, not laid out) */
 } orc_params2d;
 
 typedef struct orc_params3d {

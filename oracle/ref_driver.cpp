@@ -41,7 +41,8 @@ S2::Parameters to2d(const orc_params2d &o) {
     p.externalAbsorption = o.external_absorption;
     p.receiverStart = o.receiver_start, p.receiverCount = o.receiver_count, p.receiverStride = o.receiver_stride;
     p.receiverDepth = o.receiver_depth, p.sampleIntervalMs = o.sample_interval_ms;
-    p.autoSurfaceReceivers = false;
+    p.autoSurfaceReceivers = o.auto_surface_receivers;
+    p.receiverTargetCount = o.receiver_target_count;
     p.layered = o.layered;
     p.layers.clear();
     for (int l = 0; l < o.n_layers; ++l)
@@ -322,5 +323,74 @@ int ref_segy_trace(void *h, int i, double *pos, double *src, double *start, int6
     *start = t.startTime, *count = (int64_t)t.samples.size();
     if (samples) std::copy(t.samples.begin(), t.samples.end(), samples);
     return 1;
+}
+} // extern "C"
+
+// ---- VTS: the reference's own model document writer/reader and wavefield snapshot writer
+//      (include/Seis2D/ModelIo2D.hpp:8-13, include/Seis2D/CurvilinearSolver2D.hpp:124-140)
+#include "Seis2D/ModelIo2D.hpp"
+extern "C" {
+int ref2d_write_model_vts(void *h, const char *path, int threads, int device, char *error, int len) {
+    S3::ComputeSettings3D settings;
+    settings.cpuThreads = threads, settings.cudaDevice = device;
+    std::string e;
+    const bool ok = S2::writeModelVts(*static_cast<S2::Simulation *>(h), path, settings, e);
+    copyError(e, error, len);
+    return ok;
+}
+// Fills *out (surface samples copied to `surface`, at most `capacity` pairs) and info[4] = backend kind,
+// compute.cpuThreads, compute.cudaDevice, referenceDt.
+int ref_model_vts_read(const char *path, orc_params2d *out, double *surface, int capacity, double *info, char *error,
+                       int len) {
+    S2::ModelDocument d;
+    std::string e;
+    if (!S2::readModelVts(path, d, e)) {
+        copyError(e, error, len);
+        return 0;
+    }
+    const auto &p = d.parameters;
+    orc_params2d o{};
+    o.precision = p.precision == S2::Precision::Float32;
+    o.nx = p.nx, o.nz = p.nz, o.spacing = p.spacing, o.vp = p.vp, o.vs = p.vs, o.density = p.density;
+    o.relief = p.relief, o.wavelength = p.wavelength, o.flatten_depth = p.flattenDepth, o.frequency = p.frequency;
+    o.duration = p.duration, o.cfl = p.cfl, o.force_per_length = p.forcePerLength;
+    o.source_x = p.sourceX, o.source_z = p.sourceZ, o.max_steps = p.maxSteps;
+    o.source_kind = p.source.kind == SeisWave::SourceKind::IsotropicMoment;
+    for (int c = 0; c < 3; ++c) o.source_direction[c] = p.source.direction[c];
+    o.moment_per_length = p.momentPerLength, o.source_sigma_m = p.sourceSigmaM, o.source_radius_m = p.sourceRadiusM;
+    o.absorbing = p.absorbing, o.sponge_x = p.spongeX, o.sponge_z = p.spongeZ, o.sponge_strength = p.spongeStrength;
+    o.absorber = p.absorber == S2::Absorber::CfsPml;
+    o.pml_reflection = p.cpml.targetReflection, o.pml_kappa_max = p.cpml.kappaMax, o.pml_frequency = p.cpml.frequency;
+    o.external_absorption = p.externalAbsorption;
+    o.receiver_start = p.receiverStart, o.receiver_count = p.receiverCount, o.receiver_stride = p.receiverStride;
+    o.receiver_depth = p.receiverDepth, o.sample_interval_ms = p.sampleIntervalMs;
+    o.auto_surface_receivers = p.autoSurfaceReceivers, o.receiver_target_count = p.receiverTargetCount;
+    o.layered = p.layered, o.n_layers = static_cast<int>(p.layers.size());
+    for (int l = 0; l < o.n_layers && l < ORC_MAX_LAYERS; ++l) {
+        const auto &L = p.layers[l];
+        o.layer_thickness[l] = L.thickness, o.layer_vp[l] = L.vp, o.layer_vs[l] = L.vs, o.layer_density[l] = L.density;
+        o.layer_qp[l] = L.qp, o.layer_qs[l] = L.qs;
+    }
+    const auto &a = p.attenuation;
+    o.attenuation = a.enabled, o.mechanisms = a.mechanisms, o.q_min_frequency = a.minFrequency;
+    o.q_max_frequency = a.maxFrequency, o.q_reference_frequency = a.referenceFrequency, o.qp = a.qp, o.qs = a.qs;
+    o.imported_surface = p.importedSurface, o.surface_datum = p.surfaceDatum;
+    o.n_surface = static_cast<int>(p.surfaceSamples.size());
+    for (int i = 0; i < o.n_surface && i < capacity; ++i) surface[2 * i] = p.surfaceSamples[i][0], surface[2 * i + 1] = p.surfaceSamples[i][1];
+    o.threads = d.compute.cpuThreads;
+    *out = o;
+    info[0] = static_cast<int>(d.backend), info[1] = d.compute.cpuThreads, info[2] = d.compute.cudaDevice;
+    info[3] = d.referenceDt;
+    return 1;
+}
+// components[7]: SnapshotExportOptions::components
+int ref2d_write_snapshot(void *h, const char *path, const int *components, int include_material, char *error, int len) {
+    S2::SnapshotExportOptions o;
+    for (int c = 0; c < 7; ++c) o.components[c] = components[c] != 0;
+    o.includeMaterial = include_material != 0;
+    std::string e;
+    const bool ok = S2::writeSnapshot(*static_cast<S2::Simulation *>(h), path, e, o);
+    copyError(e, error, len);
+    return ok;
 }
 } // extern "C"

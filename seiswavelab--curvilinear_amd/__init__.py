@@ -8,5 +8,7 @@ from .params import (Absorber, AbsorbingBoundarySettings3D, BackendKind, CfsPmlS
                      ReceiverArraySettings3D, SimulationParameters, SourceKind, SourceMechanism, SurfaceGrid3D)
 from .simulation import (AsyncSimulation, Batch2D, Receiver, ReceiverTrace3D, Simulation, Simulation3D,
                          writeReceivers)
-from . import native, records, workloads
+from . import modelio, native, records, workloads
 from .records import ImportedRecords, SegyExportProfile, TraceRecord, read_segy, write_segy
+from .modelio import (ComputeSettings, ModelDocument, SnapshotExportOptions, displayedSnapshotOptions, readModelVts,
+                      writeModelVts, writeSnapshot)

@@ -19,7 +19,8 @@ INCLUDE = ROOT / "include"
 LIB = CSRC / "libswl.so"
 
 HIP_SOURCES = ["engine2d.hip", "engine3d.hip", "geometry3d.hip"]
-CXX_SOURCES = ["model2d.cpp", "sim2d.cpp", "model3d.cpp", "sim3d.cpp", "batch2d.cpp", "segy.cpp", "async2d.cpp"]
+GCC_OPENMP_SOURCES = {"vts2d.cpp"}
+CXX_SOURCES = ["model2d.cpp", "sim2d.cpp", "model3d.cpp", "sim3d.cpp", "batch2d.cpp", "segy.cpp", "async2d.cpp", "vts2d.cpp"]
 
 
 def hipcc() -> str:
@@ -39,6 +40,9 @@ def _compile(src: pathlib.Path, obj: pathlib.Path, verbose: bool):
         # FP64 session construction is plain host C++: build it with g++ so libm / complex
         # arithmetic round exactly as in the reference's CPU build.
         cmd = ["g++"] + COMMON + ["-fopenmp", "-c", str(src), "-o", str(obj)]
+    elif src.name in GCC_OPENMP_SOURCES:
+        # host code with OpenMP loops: one OpenMP runtime (libgomp, linked by g++) for the whole library
+        cmd = ["g++"] + COMMON + ["-fopenmp", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c", str(src), "-o", str(obj)]
     else:
         flags = COMMON + (HIP_FLAGS if src.suffix == ".hip" else
                           ["-x", "c++", "-fopenmp", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"])

@@ -187,6 +187,16 @@ std::array<double, 2> Pchip::sample(double x) const {
             (6 * t * t - 6 * t) * (a - b) / h + (3 * t * t - 4 * t + 1) * m + (3 * t * t - 2 * t) * n};
 }
 
+// SeisWave::Rheology::validateSettings (src/common/Gsls.cpp:28-33)
+void validateGsls(int mechanisms, double fmin, double fmax, double fref, double qp, double qs) {
+    need(mechanisms >= 1 && mechanisms <= 3, "GSLS mechanism count must be 1..3.");
+    need(std::isfinite(fmin) && std::isfinite(fmax) && std::isfinite(fref) && fmin > 0 && fmax > fmin && fmax <= 1e6 &&
+             fmin >= .01 && fref >= fmin && fref <= fmax,
+         "GSLS requires 0.01 <= fmin < fmax <= 1e6 Hz and fref within the band.");
+    for (double q : {qp, qs})
+        need(std::isfinite(q) && q >= 5 && q <= 1e9, "Qp and Qs must be finite and in 5..1e9; disable attenuation for the elastic limit.");
+}
+
 Model2D buildModel2D(const swl_params2d &requested) {
     Model2D s;
     swl_params2d p = requested;
@@ -194,6 +204,9 @@ Model2D buildModel2D(const swl_params2d &requested) {
     // validation, CurvilinearSolver2D.cpp:161-227
     need(p.precision == 0 || p.precision == 1, "Unknown 2D calculation precision.");
     need(p.nx >= 8 && p.nz >= 8 && p.nx <= 16384 && p.nz <= 16384, "2D dimensions must be in 8..16384 (subject to RAM/VRAM).");
+    need(p.receiver_target_count >= 0 && p.receiver_target_count <= 1024, "Desired surface receiver count must be 0..1024.");
+    need(!p.auto_surface_receivers,
+         "Automatic surface receiver layout is not provided by this library; set the receiver line explicitly.");
     for (double v : {p.spacing, p.relief, p.wavelength, p.flatten_depth, p.frequency, p.duration, p.cfl,
                      p.force_per_length, p.moment_per_length, p.sponge_strength, p.surface_datum, p.sample_interval_ms})
         need(std::isfinite(v), "2D parameters must be finite.");
@@ -247,15 +260,7 @@ Model2D buildModel2D(const swl_params2d &requested) {
     std::vector<GslsFit> fits;
     std::vector<GslsMaterial> rheology;
     if (p.attenuation) {
-        need(p.mechanisms >= 1 && p.mechanisms <= 3, "GSLS mechanism count must be 1..3.");
-        need(std::isfinite(p.q_min_frequency) && std::isfinite(p.q_max_frequency) && std::isfinite(p.q_reference_frequency) &&
-                 p.q_min_frequency > 0 && p.q_max_frequency > p.q_min_frequency && p.q_max_frequency <= 1e6 &&
-                 p.q_min_frequency >= .01 && p.q_reference_frequency >= p.q_min_frequency &&
-                 p.q_reference_frequency <= p.q_max_frequency,
-             "GSLS requires 0.01 <= fmin < fmax <= 1e6 Hz and fref within the band.");
-        for (double q : {p.qp, p.qs})
-            need(std::isfinite(q) && q >= 5 && q <= 1e9,
-                 "Qp and Qs must be finite and in 5..1e9; disable attenuation for the elastic limit.");
+        validateGsls(p.mechanisms, p.q_min_frequency, p.q_max_frequency, p.q_reference_frequency, p.qp, p.qs);
         for (int l = 0; l < materials; ++l) {
             const double qp = p.layered ? p.layer_qp[l] : p.qp, qs = p.layered ? p.layer_qs[l] : p.qs;
             const double vp = p.layered ? p.layer_vp[l] : p.vp, vs = p.layered ? p.layer_vs[l] : p.vs,
@@ -334,6 +339,7 @@ Model2D buildModel2D(const swl_params2d &requested) {
     s.normals.assign(2 * count, 0);
     s.mechanisms = p.attenuation ? p.mechanisms : 0;
     for (int l = 0; l < s.mechanisms; ++l) s.deltaP[l].assign(count, 0), s.deltaS[l].assign(count, 0);
+    if (p.attenuation) s.material.assign(count, 0);
     s.minJacobian = std::numeric_limits<double>::max();
     double maxVp = 0, minVs = std::numeric_limits<double>::max(), maxSigma = 0;
     // metric / material pass (:339-392)
@@ -391,6 +397,7 @@ Model2D buildModel2D(const swl_params2d &requested) {
                 }
                 s.mu[q] = m.sUnrelaxed;
                 s.lambda[q] = m.pUnrelaxed - 2 * m.sUnrelaxed;
+                s.material[q] = static_cast<uint8_t>(material);
                 for (int l = 0; l < s.mechanisms; ++l) s.deltaP[l][q] = m.deltaP[l], s.deltaS[l][q] = m.deltaS[l];
                 speedMax = std::sqrt(m.pUnrelaxed / rho);
                 speedMin = std::sqrt(m.sRelaxed / rho);

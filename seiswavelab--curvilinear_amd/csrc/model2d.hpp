@@ -55,10 +55,14 @@ struct Model2D {
     std::vector<int64_t> probeIndex;
     size_t recordReserve = 0;
     std::vector<double> surfaceSamples; // keeps imported samples alive for `logical`
+    std::vector<uint8_t> material;      // layer index per node (attenuating models: selects the node's Qp/Qs)
+    double qp(size_t q) const { return logical.layered ? logical.layer_qp[material[q]] : logical.qp; }
+    double qs(size_t q) const { return logical.layered ? logical.layer_qs[material[q]] : logical.qs; }
 };
 
 // Throws std::runtime_error with the reference's messages for invalid input.
 Model2D buildModel2D(const swl_params2d &requested);
+void validateGsls(int mechanisms, double fmin, double fmax, double fref, double qp, double qs);
 swl_mesh2d_desc meshDescriptor(const Model2D &m);
 double lerpExact(double a, double b, double t); // C++20 std::lerp contract (P0811R3)
 

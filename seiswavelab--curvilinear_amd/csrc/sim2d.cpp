@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <filesystem>
 #include <fstream>
 #include <iomanip>
 #include <locale>
@@ -186,7 +187,11 @@ int swl_sim2d_read_probes(const swl_sim2d *s, const int32_t *xz, int count, doub
 int swl_sim2d_write_receivers(const swl_sim2d *s, const char *path) {
     return swl::guarded([&] {
         // receiver CSV with provenance header (CurvilinearSolver2D.cpp:811-854)
-        std::ofstream out(path);
+        swl::require(s != nullptr, "Initialize a simulation before exporting traces.");
+        const std::filesystem::path file(path ? path : "");
+        swl::require(!file.empty() && !file.filename().empty(), "Choose a receiver CSV file name.");
+        if (!file.parent_path().empty()) std::filesystem::create_directories(file.parent_path());
+        std::ofstream out(file);
         swl::require(static_cast<bool>(out), "Cannot create 2D receiver CSV.");
         out.imbue(std::locale::classic());
         const auto &m = s->model;

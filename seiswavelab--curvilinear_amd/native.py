@@ -81,6 +81,12 @@ SIGNATURES = {
     "swl_segy_records_trace": (C.c_int, [vp, C.c_int, vp, vp, P(C.c_double), P(C.c_int64)]),
     "swl_segy_records_samples": (C.c_int, [vp, C.c_int, vp]),
     "swl_sim2d_write_segy": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_double]),
+    "swl_sim2d_write_model_vts": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int]),
+    "swl_model_vts_read": (C.c_int, [C.c_char_p, P(vp)]),
+    "swl_model_doc2d_destroy": (None, [vp]),
+    "swl_model_doc2d_params": (C.c_int, [vp, vp]),
+    "swl_model_doc2d_info": (C.c_int, [vp, vp]),
+    "swl_sim2d_write_snapshot": (C.c_int, [vp, C.c_char_p, vp, C.c_int]),
     "swl_async2d_create": (C.c_int, [P(vp)]),
     "swl_async2d_destroy": (None, [vp]),
     "swl_async2d_initialize": (C.c_int, [vp, vp, C.c_int, C.c_int]),

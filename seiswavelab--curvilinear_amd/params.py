@@ -132,6 +132,8 @@ class Parameters:
     receiverCount: int = 96
     receiverStride: int = 2
     receiverDepth: int = 1
+    autoSurfaceReceivers: bool = False  # interactive survey policy; stored with the model, not laid out here
+    receiverTargetCount: int = 0
     sampleIntervalMs: float = 0.0
     layered: bool = False
     attenuation: GslsSettings = field(default_factory=GslsSettings)
@@ -174,6 +176,7 @@ class CParams2D(C.Structure):
         ("node_vp", C.POINTER(C.c_double)), ("node_vs", C.POINTER(C.c_double)),
         ("node_density", C.POINTER(C.c_double)),
         ("threads", C.c_int),
+        ("auto_surface_receivers", C.c_int), ("receiver_target_count", C.c_int),
     ]
 
 
@@ -231,7 +234,33 @@ def to_c(p: Parameters):
             keep.append(arr)
             setattr(c, cname, ptr)
     c.threads = p.cpuThreads
+    c.auto_surface_receivers, c.receiver_target_count = int(p.autoSurfaceReceivers), int(p.receiverTargetCount)
     return c, keep
+
+
+def from_c(c: CParams2D) -> Parameters:
+    """Inverse of ``to_c`` for a parameter block without per-node arrays (surface samples are copied)."""
+    n = max(int(c.n_surface), 0)
+    surface = [(float(c.surface_xz[2 * i]), float(c.surface_xz[2 * i + 1])) for i in range(n)] if n else []
+    layers = [Layer(c.layer_thickness[i], c.layer_vp[i], c.layer_vs[i], c.layer_density[i], c.layer_qp[i],
+                    c.layer_qs[i]) for i in range(int(c.n_layers))]
+    return Parameters(
+        precision=Precision(c.precision), nx=c.nx, nz=c.nz, spacing=c.spacing, vp=c.vp, vs=c.vs,
+        density=c.density, relief=c.relief, wavelength=c.wavelength, flattenDepth=c.flatten_depth,
+        frequency=c.frequency, duration=c.duration, cfl=c.cfl, forcePerLength=c.force_per_length,
+        sourceX=c.source_x, sourceZ=c.source_z, maxSteps=c.max_steps,
+        source=SourceMechanism(SourceKind(c.source_kind), tuple(float(v) for v in c.source_direction)),
+        momentPerLength=c.moment_per_length, sourceSigmaM=c.source_sigma_m, sourceRadiusM=c.source_radius_m,
+        absorbing=bool(c.absorbing), spongeX=c.sponge_x, spongeZ=c.sponge_z, spongeStrength=c.sponge_strength,
+        absorber=Absorber(c.absorber), cpml=CfsPmlSettings(c.pml_reflection, c.pml_kappa_max, c.pml_frequency),
+        externalAbsorption=bool(c.external_absorption), receiverStart=c.receiver_start,
+        receiverCount=c.receiver_count, receiverStride=c.receiver_stride, receiverDepth=c.receiver_depth,
+        autoSurfaceReceivers=bool(c.auto_surface_receivers), receiverTargetCount=c.receiver_target_count,
+        sampleIntervalMs=c.sample_interval_ms, layered=bool(c.layered),
+        attenuation=GslsSettings(bool(c.attenuation), c.mechanisms, c.q_min_frequency, c.q_max_frequency,
+                                 c.q_reference_frequency, c.qp, c.qs),
+        importedSurface=bool(c.imported_surface), surfaceDatum=c.surface_datum, surfaceSamples=surface,
+        layers=layers, cpuThreads=c.threads)
 
 
 @dataclass

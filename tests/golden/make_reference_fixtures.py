@@ -1,10 +1,12 @@
-"""Writes tests/golden/reference_*.npz: input/output vectors of the reference's own CPU solver.
+"""Writes tests/golden/reference_*.npz, segy_*.sgy and vts_*.vts.gz: outputs of the reference's own code.
 
 The reference's 2D and 3D CPU paths are compiled from its sources into oracle/_ref/libswlref.so
 (oracle/Makefile, target `ref`).  This script runs that build on the small cases of
 tests/test_oracle_2d.py (REF_CASES) and tests/test_oracle_3d.py (CASES) and keeps, per case:
 dt, total steps, the energy and peak energy after every step, the receiver traces, and per final
 field its L2 norm, sum and max |value|.  The inputs are the case definitions in those test files.
+It also runs the reference's SEG-Y writer on a synthetic record set, and its model-document and
+wavefield-snapshot writers on the small VTS cases of tests/reference_fixtures.py (gzipped text).
 
     python tests/golden/make_reference_fixtures.py      (needs oracle/_ref/libswlref.so)
 """
@@ -17,8 +19,9 @@ HERE = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(HERE.parent))
 import conftest  # noqa: E402,F401  registers the package
 from cpu_sessions import load  # noqa: E402
-from reference_fixtures import (FIXTURES_2D, SEGY_VARIANTS, fixture_path, run_oracle_2d, run_oracle_3d,  # noqa: E402
-                                segy_fixture_path, segy_records)
+from reference_fixtures import (FIXTURES_2D, SEGY_VARIANTS, VTS_ALL_COMPONENTS, VTS_MODEL_VARIANTS,  # noqa: E402
+                                VTS_SNAPSHOT_CASES, fixture_path, run_oracle_2d, run_oracle_3d, segy_fixture_path,
+                                segy_records, vts_case, vts_fixture_path)
 from test_oracle_2d import ref_case  # noqa: E402
 from test_oracle_3d import CASES, case  # noqa: E402
 
@@ -40,6 +43,36 @@ def write_segy_fixtures():
         assert ok, err.value
 
 
+def write_vts_fixtures():
+    """The reference's writeModelVts (ComputeSettings3D{1, -1}, CPU session) for each model variant, and its
+    writeSnapshot (all seven components and the materials) at step 0 and after VTS_SNAPSHOT_CASES steps."""
+    import ctypes as C
+    import gzip
+    import shutil
+    import tempfile
+    from cpu_sessions import CpuSession2D
+    lib = load("ref")
+    lib.ref2d_write_model_vts.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_char_p, C.c_int]
+    lib.ref2d_write_snapshot.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_int, C.c_char_p, C.c_int]
+    err = C.create_string_buffer(512)
+    comps = (C.c_int * 7)(*VTS_ALL_COMPONENTS)
+    with tempfile.TemporaryDirectory() as tmp:
+        def keep(src, dst):
+            with open(src, "rb") as a, gzip.GzipFile(dst, "wb", mtime=0) as b:
+                shutil.copyfileobj(a, b)
+        for name in VTS_MODEL_VARIANTS:
+            s = CpuSession2D("ref", vts_case(name))
+            assert lib.ref2d_write_model_vts(C.c_void_p(s.h), f"{tmp}/m.vts".encode(), 1, -1, err, 512), err.value
+            keep(f"{tmp}/m.vts", vts_fixture_path(name))
+        for name, steps in VTS_SNAPSHOT_CASES.items():
+            s = CpuSession2D("ref", vts_case(name))
+            for step in (0, steps):
+                ok, e = s.advance(step - s.step)
+                assert ok, e
+                assert lib.ref2d_write_snapshot(C.c_void_p(s.h), f"{tmp}/s.vts".encode(), comps, 1, err, 512), err.value
+                keep(f"{tmp}/s.vts", vts_fixture_path(name, step))
+
+
 def main():
     if load("ref") is None:
         sys.exit("oracle/_ref/libswlref.so is missing: build it with `make -C oracle ref` first")
@@ -50,7 +83,8 @@ def main():
     for name in CASES:
         np.savez_compressed(fixture_path("3d", name), **run_oracle_3d("ref", case(name)))
     write_segy_fixtures()
-    for f in sorted(list(HERE.glob("reference_*.npz")) + list(HERE.glob("segy_*.sgy"))):
+    write_vts_fixtures()
+    for f in sorted(list(HERE.glob("reference_*.npz")) + list(HERE.glob("segy_*.sgy")) + list(HERE.glob("vts_*.gz"))):
         print(f.name, f.stat().st_size)
 
 

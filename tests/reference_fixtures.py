@@ -90,3 +90,92 @@ def segy_records(seed=17, n=5, count=97):
 
 def segy_fixture_path(name):
     return GOLDEN / f"segy_{name}.sgy"
+
+
+# ---- VTS model documents and wavefield snapshots (tests/golden/vts_*.vts.gz, the reference's own writers) ----
+VTS_MODEL_VARIANTS = ["v2_sponge", "v2_gsls_layered_surface", "v3_pml", "v4_external", "v5_fp32_gsls",
+                      "v6_physical", "v7_target_moment"]
+VTS_SNAPSHOT_CASES = {"v2_gsls_layered_surface": 30, "v4_external": 30, "v5_fp32_gsls": 30}  # name: steps
+VTS_ALL_COMPONENTS = [1, 1, 1, 1, 1, 1, 1]
+
+
+def vts_case(name):
+    """Small (24 x 20) sessions, one per schema feature of the model document."""
+    from seiswavelab_curvilinear_amd.params import Absorber, Layer, Parameters, SourceKind
+    p = Parameters(nx=24, nz=20, spacing=2.5, relief=2, wavelength=40, sourceZ=8, receiverStart=2,
+                   receiverCount=4, receiverStride=4, receiverDepth=2, duration=.01, maxSteps=40, spongeX=4,
+                   spongeZ=4, cpuThreads=1)
+    p.source.direction = (1, 0, 2)
+    if name == "v2_gsls_layered_surface":
+        p.attenuation.enabled, p.attenuation.mechanisms = True, 2
+        p.layered = True
+        p.layers = [Layer(12, 1800, 900, 2000, 40, 25), Layer(0, 3000, 1700, 2300, 80, 50)]
+        p.importedSurface, p.surfaceDatum = True, 500
+        p.surfaceSamples = [(0, 499), (20, 501.5), (40, 500.25), (60, 499.5)]
+        p.sampleIntervalMs = .25
+    elif name == "v3_pml":
+        p.absorber = Absorber.CfsPml
+    elif name == "v4_external":
+        p.absorber, p.externalAbsorption = Absorber.CfsPml, True
+    elif name == "v5_fp32_gsls":
+        p.precision, p.absorber = Precision.Float32, Absorber.CfsPml
+        p.attenuation.enabled = True
+    elif name == "v6_physical":
+        p.sourceSigmaM, p.sourceRadiusM = 2.0, 5.0
+    elif name == "v7_target_moment":
+        p.receiverTargetCount = 12
+        p.source.kind = SourceKind.IsotropicMoment
+    return p
+
+
+def vts_fixture_path(name, step=None):
+    return GOLDEN / (f"vts_model_{name}.vts.gz" if step is None else f"vts_snapshot_{name}_step{step}.vts.gz")
+
+
+def vts_text(path):
+    import gzip
+    opener = gzip.open if str(path).endswith(".gz") else open
+    with opener(path, "rt", encoding="ascii", newline="") as f:
+        return f.read()
+
+
+def parse_vts(text):
+    """Independent reading of a VTS file of either schema: {"field": {name: [values]}, "point": {name: array},
+    "coordinates": (n, 3) array, "order": [field array names in file order]}."""
+    import re
+    out = {"field": {}, "point": {}, "coordinates": None, "order": []}
+    field_end = text.index("</FieldData>")
+    for m in re.finditer(r"<DataArray ([^>]*)>([^<]*)</DataArray>", text):
+        attrs = dict(re.findall(r'(\w+)="([^"]*)"', m.group(1)))
+        values = np.array([float(v) for v in m.group(2).split()])
+        if m.start() < field_end:
+            out["field"][attrs["Name"]] = values
+            out["order"].append(attrs["Name"])
+        elif attrs.get("NumberOfComponents") == "3":
+            out["coordinates"] = values.reshape(-1, 3)
+        else:
+            out["point"][attrs["Name"]] = values
+    return out
+
+
+def fnv1a64(data: bytes, h=0xcbf29ce484222325):
+    """FNV-1a 64-bit, byte by byte (Fowler-Noll-Vo)."""
+    for b in data:
+        h = ((h ^ b) * 0x100000001b3) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def model_checksum(doc):
+    """The document checksum recomputed from the text: names (NUL-terminated) and values (IEEE-754 little
+    endian, -0 as +0) of every metadata array in file order except ModelChecksum, then the six point arrays
+    in file order, then Coordinates (x, 0, z per node)."""
+    import struct
+    h = 0xcbf29ce484222325
+    number = lambda v: struct.pack("<d", 0.0 if v == 0 else float(v))  # noqa: E731
+    for name in doc["order"]:
+        if name == "ModelChecksum":
+            continue
+        h = fnv1a64(name.encode() + b"\0" + b"".join(number(v) for v in doc["field"][name]), h)
+    for name, values in doc["point"].items():
+        h = fnv1a64(name.encode() + b"\0" + b"".join(number(v) for v in values), h)
+    return fnv1a64(b"Coordinates\0" + b"".join(number(v) for v in doc["coordinates"].ravel()), h)
