@@ -94,7 +94,7 @@ struct Runner2D {
     // streams (longest-first onto the least-loaded lane), so small shots' launches overlap on the device.
     // Two lanes measured best for shots of ~0.1-1 M nodes: two shots' working sets (~100 MB each for
     // GSLS-3 FP32) still fit the 256 MB Infinity Cache together, four do not.
-    int fused = 0, lanes = 2, laneSteps = 8;
+    int fused = 0, lanes = 2, laneSteps = 32;
     std::vector<hipStream_t> laneStreams;
     PinnedBuffer verdicts;
     long long laneRuns = 0;

@@ -484,7 +484,54 @@ template <class R> __device__ inline void vstore(R *p, typename Lane<R>::V v, in
 // true divisions), so FP64 runs reproduce the reference to the bit.  FP32 is held to a tolerance, not
 // to the bit, so its interior fuses multiply-adds and replaces repeated divisions by one reciprocal.
 
-template <class R, int MECH> constexpr bool kRebuildY = kFast<R> && MECH == 0;
+// Stage combine of one interior field at offset o.  FP32: the accumulator holds
+// Y = U^n + dt/6 k1 + dt/3 k2 + dt/3 k3, so the last stage reads Y and the stage input only (the compulsory 3S
+// traffic), not U^n as well; stage 1 does not store Y1 = U^n + dt/6 k1: stage 2 rebuilds it from what it reads
+// anyway, Y1 = U^n + (U2 - U^n) / 3 (U2 = U^n + dt/2 k1), which saves one write and one read of S.  FP64: the
+// reference's accumulator form.  vin is the field's stage input (which is the base in stage 1) when
+// `haveInput`; k holds the field's rates; the new values go to nxt.
+template <class R>
+__device__ __forceinline__ void combine_field(const StageArgs<R> &A, size_t o, bool haveInput, const typename Lane<R>::V &vin,
+                                              const R (&k)[Lane<R>::N], R (&nxt)[Lane<R>::N]) {
+    using V = typename Lane<R>::V;
+    constexpr int NV = Lane<R>::N;
+    V vo, vacc;
+    if constexpr (kFast<R>) {
+        const V vb = A.kind == 2 ? V{} : A.kind == 0 && haveInput ? vin : vload(A.base + o);
+        V vy;
+        if (A.kind == 0) {
+            vy = vb;
+        } else if (A.ystage) {
+#pragma unroll
+            for (int j = 0; j < NV; ++j) vy[j] = mad(vin[j] - vb[j], (R)(1.0 / 3.0), vb[j]);
+        } else {
+            vy = vload(A.acc + o);
+        }
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            if (A.kind == 2) {
+                nxt[j] = mad(A.wy, k[j], vy[j]);
+            } else {
+                nxt[j] = mad(A.w, k[j], vb[j]);
+                vacc[j] = mad(A.wy, k[j], vy[j]);
+            }
+            vo[j] = nxt[j];
+        }
+    } else {
+        const V vb = A.kind == 0 && haveInput ? vin : vload(A.base + o); // stage 1: base is the input
+        V va;
+        if (A.kind != 0) va = vload(A.acc + o);
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            R acc = A.kind == 0 ? (R)0 : va[j];
+            nxt[j] = combine(A.kind, vb[j], k[j], A.w, acc);
+            vo[j] = nxt[j];
+            vacc[j] = acc;
+        }
+    }
+    vstore(A.out + o, vo, A.nt);
+    if (A.kind == 1 || (A.kind == 0 && !kFast<R>)) vstore(A.acc + o, vacc, A.nt);
+}
 // LDS staging of the stencil input (the `tile` variant): per field, the block's 4 rows plus the 3 + 1
 // halo rows of the z-stencil (8 rows) and HV halo vectors on each side of the 64 lane vectors are loaded
 // once into LDS (double-buffered over the fields); the lanes then read their x- and z-taps from LDS.
@@ -615,7 +662,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
         // layered models: materials depend on the row k only (grid-conformal layers), read per row
         const V vrho = m.by_row ? V(m.rho_k[k]) : vload(m.rho + q), vlam = m.by_row ? V(m.lam_k[k]) : vload(m.lam + q),
                 vmu = m.by_row ? V(m.mu_k[k]) : vload(m.mu + q);
-        R kv[F][NV];
+        R kv[MECH ? 6 : 5][NV];
         const bool src = m.si1 >= 0 && k >= m.sk0 && k <= m.sk1 && i0 + NV - 1 >= m.si0 && i0 <= m.si1;
 #pragma unroll
         for (int j = 0; j < NV; ++j) {
@@ -667,86 +714,74 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
             kv[4][j] = mu * (dz[0] + dx[1]);
             if constexpr (MECH > 0) kv[5][j] = lambda * div;
         }
+        // GSLS, in three passes so that few rates are live at once: (1) the stress rates take each mechanism's
+        // memories in turn (the reference's order); (2) velocities and stresses are combined, stored and
+        // inspected; (3) mechanism by mechanism, the memories' rates are formed and their four fields combined.
+        // Pass 3 reads the memories again, from the caches, instead of holding them in registers.
+        if constexpr (MECH > 0) {
+#pragma unroll
+            for (int l = 0; l < MECH; ++l)
+#pragma unroll
+                for (int f = 0; f < 4; ++f) {
+                    const V mv = vload(A.in + (size_t)(6 + 4 * l + f) * cnt + q);
+#pragma unroll
+                    for (int j = 0; j < NV; ++j) kv[2 + f][j] -= mv[j];
+                }
+        }
+        {
+            R nxt[5][NV];
+#pragma unroll
+            for (int f = 0; f < (MECH ? 6 : 5); ++f) {
+                const size_t o = (size_t)f * cnt + q;
+                // stage input: the stencil centre; Syy's only where FP32 stage 2 rebuilds Y1 from it
+                const V vin = f < 5 ? centre[f < 5 ? f : 0] : (kFast<R> && A.ystage ? vload(A.in + o) : V{});
+                R nx[NV];
+                combine_field<R>(A, o, f < 5, vin, kv[f], nx);
+                if (A.kind == 2)
+#pragma unroll
+                    for (int j = 0; j < NV; ++j) {
+                        if (!isfinite(nx[j])) p.bad = 1;
+                        if (f < 5) nxt[f < 5 ? f : 0][j] = nx[j];
+                    }
+            }
+            if (A.kind == 2) {
+#pragma unroll
+                for (int j = 0; j < NV; ++j) {
+                    R u[5];
+#pragma unroll
+                    for (int f = 0; f < 5; ++f) u[f] = nxt[f][j];
+                    node_stats<R, kFast<R>>(vrho[j], vlam[j], vmu[j], jac[j], h, u, false, nullptr, p);
+                }
+            }
+        }
         if constexpr (MECH > 0) {
 #pragma unroll
             for (int l = 0; l < MECH; ++l) {
                 const int kk = 6 + 4 * l;
                 const V vdp = m.by_row ? V(m.dP_k[l][k]) : vload(m.dP[l] + q), vds = m.by_row ? V(m.dS_k[l][k]) : vload(m.dS[l] + q);
-                V mem[4];
 #pragma unroll
-                for (int f = 0; f < 4; ++f) mem[f] = vload(A.in + (size_t)(kk + f) * cnt + q);
+                for (int f = 0; f < 4; ++f) {
+                    const size_t o = (size_t)(kk + f) * cnt + q;
+                    const V mem = vload(A.in + o);
+                    R rate[NV];
 #pragma unroll
-                for (int j = 0; j < NV; ++j) {
-                    const R dp = vdp[j], ds = vds[j], dl = dp - 2 * ds;
-                    const R dx0 = xi[0][j] + a[j] * eta[0][j], dx1 = xi[1][j] + a[j] * eta[1][j];
-                    const R dz0 = b[j] * eta[0][j], dz1 = b[j] * eta[1][j], div = dx0 + dz1;
-                    const R drive[4] = {dl * div + 2 * ds * dx0, dl * div + 2 * ds * dz1, ds * (dz0 + dx1), dl * div};
-                    kv[2][j] -= mem[0][j];
-                    kv[3][j] -= mem[1][j];
-                    kv[4][j] -= mem[2][j];
-                    kv[5][j] -= mem[3][j];
-#pragma unroll
-                    for (int f = 0; f < 4; ++f) kv[kk + f][j] = m.rates[l] * (drive[f] - mem[f][j]);
-                }
-            }
-        }
-        R nxt[F][NV];
-#pragma unroll
-        for (int f = 0; f < F; ++f) {
-            const size_t o = (size_t)f * cnt + q;
-            V vo, vacc;
-            if constexpr (kFast<R>) {
-                // FP32: the accumulator holds Y = U^n + dt/6 k1 + dt/3 k2 + dt/3 k3, so the last stage
-                // reads Y and the stage input only (the compulsory 3S traffic), not U^n as well.  Stage 1
-                // does not store Y1 = U^n + dt/6 k1: stage 2 rebuilds it from what it reads anyway,
-                // Y1 = U^n + (U2 - U^n) / 3 (U2 = U^n + dt/2 k1), which saves one write and one read of S.
-                // Elastic only: for GSLS the extra live values cost more occupancy than the bytes save.
-                const V vb = A.kind == 2 ? V{} : A.kind == 0 && f < 5 ? centre[f < 5 ? f : 0] : vload(A.base + o);
-                V vy;
-                if (A.kind == 0) {
-                    vy = vb;
-                } else if (kRebuildY<R, MECH> && A.ystage) {
-                    const V vin = f < 5 ? centre[f < 5 ? f : 0] : vload(A.in + o);
-#pragma unroll
-                    for (int j = 0; j < NV; ++j) vy[j] = mad(vin[j] - vb[j], (R)(1.0 / 3.0), vb[j]);
-                } else {
-                    vy = vload(A.acc + o);
-                }
-#pragma unroll
-                for (int j = 0; j < NV; ++j) {
-                    if (A.kind == 2) {
-                        nxt[f][j] = mad(A.wy, kv[f][j], vy[j]);
-                    } else {
-                        nxt[f][j] = mad(A.w, kv[f][j], vb[j]);
-                        vacc[j] = mad(A.wy, kv[f][j], vy[j]);
+                    for (int j = 0; j < NV; ++j) {
+                        const R dp = vdp[j], ds = vds[j], dl = dp - 2 * ds;
+                        const R dx0 = xi[0][j] + a[j] * eta[0][j], dx1 = xi[1][j] + a[j] * eta[1][j];
+                        const R dz0 = b[j] * eta[0][j], dz1 = b[j] * eta[1][j], div = dx0 + dz1;
+                        const R drive = f == 0 ? dl * div + 2 * ds * dx0
+                                        : f == 1 ? dl * div + 2 * ds * dz1
+                                        : f == 2 ? ds * (dz0 + dx1)
+                                                 : dl * div;
+                        rate[j] = m.rates[l] * (drive - mem[j]);
                     }
-                    vo[j] = nxt[f][j];
-                }
-            } else {
-                const V vb = A.kind == 0 && f < 5 ? centre[f < 5 ? f : 0] : vload(A.base + o); // stage 1: base is the input
-                V va;
-                if (A.kind != 0) va = vload(A.acc + o);
+                    R nx[NV];
+                    combine_field<R>(A, o, true, mem, rate, nx);
+                    if (A.kind == 2)
 #pragma unroll
-                for (int j = 0; j < NV; ++j) {
-                    R acc = A.kind == 0 ? (R)0 : va[j];
-                    nxt[f][j] = combine(A.kind, vb[j], kv[f][j], A.w, acc);
-                    vo[j] = nxt[f][j];
-                    vacc[j] = acc;
+                        for (int j = 0; j < NV; ++j)
+                            if (!isfinite(nx[j])) p.bad = 1;
                 }
-            }
-            vstore(A.out + o, vo, A.nt);
-            if (A.kind == 1 || (A.kind == 0 && !kRebuildY<R, MECH>)) vstore(A.acc + o, vacc, A.nt);
-        }
-        if (A.kind == 2) {
-#pragma unroll
-            for (int j = 0; j < NV; ++j) {
-                R u[5];
-#pragma unroll
-                for (int f = 0; f < F; ++f) {
-                    if (!isfinite(nxt[f][j])) p.bad = 1;
-                    if (f < 5) u[f] = nxt[f][j];
-                }
-                node_stats<R, kFast<R>>(vrho[j], vlam[j], vmu[j], jac[j], h, u, false, nullptr, p);
             }
         }
     }
