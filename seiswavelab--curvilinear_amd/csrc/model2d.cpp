@@ -197,7 +197,7 @@ void validateGsls(int mechanisms, double fmin, double fmax, double fref, double 
         need(std::isfinite(q) && q >= 5 && q <= 1e9, "Qp and Qs must be finite and in 5..1e9; disable attenuation for the elastic limit.");
 }
 
-Model2D buildModel2D(const swl_params2d &requested) {
+Model2D buildModel2D(const swl_params2d &requested, int device) {
     Model2D s;
     swl_params2d p = requested;
     const swl_params2d logical = requested;
@@ -352,17 +352,25 @@ Model2D buildModel2D(const swl_params2d &requested) {
     }
     s.surfaceDepth.resize(gnx), s.surfaceSlope.resize(gnx);
     for (int i = 0; i < gnx; ++i) s.surfaceDepth[i] = surface[i][0], s.surfaceSlope[i] = surface[i][1];
+    // device sessions: coordinates, J, a, b and their CFL figures (min J, max metric) in FP64 on the GPU; the
+    // first node with a bad Jacobian is reported here when the pass below reaches it
+    const bool onDevice = device >= 0;
+    const long long badNode = onDevice ? buildGeometry2DOnDevice(s, device) : -1;
     for (int k = 0; k < gnz; ++k)
         for (int i = 0; i < gnx; ++i) {
             const size_t q = static_cast<size_t>(i) + static_cast<size_t>(gnx) * k;
             const double x = (i - 3 - off) * p.spacing, eta = (k - 3) * p.spacing;
             const double zx = surface[i][1] * fade[k], j = 1 + surface[i][0] * fadeDer[k];
-            need(j > 1e-9 && std::isfinite(j), "Nonpositive curvilinear Jacobian.");
-            s.coordinates[2 * q] = x;
-            s.coordinates[2 * q + 1] = eta + surface[i][0] * fade[k];
-            s.a[q] = -zx / j;
-            s.b[q] = 1 / j;
-            s.jac[q] = j;
+            if (onDevice) {
+                need(static_cast<long long>(q) != badNode, "Nonpositive curvilinear Jacobian.");
+            } else {
+                need(j > 1e-9 && std::isfinite(j), "Nonpositive curvilinear Jacobian.");
+                s.coordinates[2 * q] = x;
+                s.coordinates[2 * q + 1] = eta + surface[i][0] * fade[k];
+                s.a[q] = -zx / j;
+                s.b[q] = 1 / j;
+                s.jac[q] = j;
+            }
             const double n = std::hypot(zx, 1.0);
             s.normals[2 * q] = -zx / n;
             s.normals[2 * q + 1] = 1 / n;
@@ -403,8 +411,10 @@ Model2D buildModel2D(const swl_params2d &requested) {
                 speedMin = std::sqrt(m.sRelaxed / rho);
             }
             if (i >= 3 && i < gnx - 3 && k >= 3 && k < gnz - 3) {
-                s.minJacobian = std::min(s.minJacobian, j);
-                s.maxMetric = std::max(s.maxMetric, std::sqrt(1 + s.a[q] * s.a[q] + s.b[q] * s.b[q]));
+                if (!onDevice) {
+                    s.minJacobian = std::min(s.minJacobian, j);
+                    s.maxMetric = std::max(s.maxMetric, std::sqrt(1 + s.a[q] * s.a[q] + s.b[q] * s.b[q]));
+                }
                 maxVp = std::max(maxVp, speedMax);
                 minVs = std::min(minVs, speedMin);
             }

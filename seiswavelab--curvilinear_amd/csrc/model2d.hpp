@@ -61,7 +61,10 @@ struct Model2D {
 };
 
 // Throws std::runtime_error with the reference's messages for invalid input.
-Model2D buildModel2D(const swl_params2d &requested);
+// device >= 0: coordinates, J, a, b and min J / max metric are built on that GPU (geometry2d.hip), bit for bit
+// the host's values; device < 0: all on the host.
+Model2D buildModel2D(const swl_params2d &requested, int device = -1);
+long long buildGeometry2DOnDevice(Model2D &m, int device);
 void validateGsls(int mechanisms, double fmin, double fmax, double fref, double qp, double qs);
 swl_mesh2d_desc meshDescriptor(const Model2D &m);
 double lerpExact(double a, double b, double t); // C++20 std::lerp contract (P0811R3)

@@ -52,7 +52,7 @@ int swl_sim2d_create(const swl_params2d *p, int backend, int device, swl_sim2d *
         swl::require(backend == 2, "Only the HIP backend (2) is provided by this library; the CPU path is the reference's own.");
         const auto begin = std::chrono::steady_clock::now();
         auto s = std::make_unique<swl_sim2d>();
-        s->model = swl::buildModel2D(*p);
+        s->model = swl::buildModel2D(*p, device);
         s->device = device;
         const auto desc = swl::meshDescriptor(s->model);
         if (swl_engine2d_create(&desc, s->model.probeIndex.data(), static_cast<int>(s->model.probeIndex.size()), device,

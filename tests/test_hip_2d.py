@@ -187,6 +187,56 @@ def test_hip_matches_reference_fixtures(name, prec):
     s.close()
 
 
+GEOMETRY_CASES_2D = CASES_2D + ["vts:v2_gsls_layered_surface", "vts:v4_external", "vts:v5_fp32_gsls"]
+
+
+def _geometry_case(name):
+    if name.startswith("vts:"):
+        from reference_fixtures import vts_case
+        return vts_case(name[4:])
+    return ref_case(name)
+
+
+def _compare_geometry(p):
+    host = Simulation()
+    ok, err = host.initialize(p, BackendKind.Cpu)
+    assert ok, err
+    dev = Simulation()
+    ok, err = dev.initialize(p, BackendKind.Hip, 0)
+    assert ok, err
+    assert dev.dt() == host.dt() and dev.totalSteps() == host.totalSteps() and dev.recordDt() == host.recordDt()
+    for which in ("coordinates", "a", "b", "jacobian", "normals", "vp", "vs", "density", "lambda", "mu", "damping"):
+        assert np.array_equal(dev.model_array(which), host.model_array(which)), which
+    dh, dd = host.diagnostics(), dev.diagnostics()
+    for k in ("maxMetric", "minJacobian", "maxEdge", "flattenDepth", "waveDt", "dampingDt", "sourceDt", "cpmlDt",
+              "relaxationDt", "ppw"):
+        assert dh[k] == dd[k], k
+    return host, dev
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", GEOMETRY_CASES_2D)
+def test_device_geometry_equals_host_geometry_2d(name):
+    """A HIP session builds its coordinates, Jacobian, inverse metrics and the CFL figures min J / max metric on
+    the GPU in FP64 (csrc/geometry2d.hip); they must equal the host builder's (= the oracle's, = the
+    reference's) bit for bit, and so must dt, the step count and everything derived from them."""
+    print(f"[2d geometry] {name}", flush=True)
+    host, dev = _compare_geometry(_geometry_case(name))
+    host.close(), dev.close()
+
+
+@pytest.mark.gpu
+def test_device_geometry_grf_workload_2d():
+    """The 5001x2001 GRF-terrain workload (imported PCHIP surface, external PML, per-node velocities)."""
+    from seiswavelab_curvilinear_amd import workloads
+    w = workloads.grf_terrain_5001x2001(precision=Precision.Float32, steps=4)
+    print("[2d geometry] 5001x2001 workload", flush=True)
+    host, dev = _compare_geometry(w.params)
+    print(f"[2d geometry] HIP session initialisation {dev.diagnostics()['initializationMs']:.0f} ms", flush=True)
+    host.close(), dev.close()
+
+
+
 def test_session_segy_export_round_trip(tmp_path):
     """swl_sim2d_write_segy: the session's receivers as SEG-Y (Experiment2D.cpp:44-71 layout), read back
     exactly (IEEE64 profile): samples, receiver positions (x, 0, z) to the millimetre, the source node's
