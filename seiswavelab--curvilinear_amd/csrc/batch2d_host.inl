@@ -96,6 +96,8 @@ struct Runner2D {
     // GSLS-3 FP32) still fit the 256 MB Infinity Cache together, four do not.
     int fused = 0, lanes = 2, laneSteps = 32;
     std::vector<hipStream_t> laneStreams;
+    std::vector<hipEvent_t> laneEvents; // laneDepth per lane: ends of that lane's last turns (a ring)
+    int laneDepth = 2;                  // turns a lane may have enqueued ahead of the device
     PinnedBuffer verdicts;
     long long laneRuns = 0;
     int chunk = 32, useGraphs = 1, autotune = 1, fixedGrid = 0;
@@ -120,6 +122,7 @@ struct Runner2D {
         candidates = {0, 8 * cus, 4 * cus, 2 * cus};
     }
     ~Runner2D() {
+        for (auto e : laneEvents) (void)hipEventDestroy(e);
         for (auto s : laneStreams) (void)hipStreamDestroy(s);
         if (t0) (void)hipEventDestroy(t0);
         if (t1) (void)hipEventDestroy(t1);
@@ -382,6 +385,11 @@ struct Runner2D {
             check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "batch lane stream");
             laneStreams.push_back(s);
         }
+        while (laneEvents.size() < (size_t)laneDepth * laneStreams.size()) {
+            hipEvent_t e{};
+            check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "batch lane event");
+            laneEvents.push_back(e);
+        }
         struct Item {
             swl_batch_shot *shot;
             double cost;
@@ -418,6 +426,7 @@ struct Runner2D {
         std::vector<size_t> at(lanes, 0);
         std::vector<int> next(lanes, 0);
         std::vector<size_t> base(lanes, 0);
+        std::vector<long long> turns(lanes, 0);
         for (int l = 1; l < lanes; ++l) base[l] = base[l - 1] + plan[l - 1].size();
         for (bool any = true; any;) {
             any = false;
@@ -427,12 +436,19 @@ struct Runner2D {
                 swl_batch_shot *sh = plan[l][at[l]];
                 V *v = vs + base[l] + at[l];
                 const hipStream_t s = laneStreams[l];
+                // at most `laneDepth` turns of this lane in flight: the queues stay short, so set-up work that
+                // shares a hardware queue with a lane (sessions being built on other threads) is not held
+                // behind a whole chunk of stages
+                hipEvent_t &done = laneEvents[(size_t)laneDepth * l + turns[l] % laneDepth];
+                if (turns[l] >= laneDepth) check(hipEventSynchronize(done), "batch lane turn");
                 const int j0 = next[l], j1 = std::min(sh->count, j0 + laneSteps);
                 sh->engine->with([&](auto &e) {
                     if (j0 == 0) e.run_begin(s, sh->peak_energy);
                     e.run_steps(s, sh->first_step, j0, j1, sh->dt, sh->frequency, sh->scale);
                     if (j1 == sh->count) e.run_end(s, reinterpret_cast<typename std::decay_t<decltype(e)>::Verdict *>(v));
                 });
+                check(hipEventRecord(done, s), "batch lane turn");
+                ++turns[l];
                 next[l] = j1;
                 if (j1 == sh->count) ++at[l], next[l] = 0;
             }
@@ -496,11 +512,21 @@ int swl_batch2d_runner_set_option(swl_batch2d_runner *r, const char *name, int v
         else if (key == "lanes") {
             swl::require(value >= 1 && value <= 16, "lanes must be in 1..16");
             if (value != x.lanes) {
+                for (auto e : x.laneEvents) (void)hipEventDestroy(e);
+                x.laneEvents.clear();
                 for (auto s : x.laneStreams) (void)hipStreamDestroy(s);
                 x.laneStreams.clear();
                 x.lanes = value;
             }
         } else if (key == "lane_steps") x.laneSteps = std::clamp(value, 1, 4096);
+        else if (key == "lane_depth") {
+            const int depth = std::clamp(value, 1, 64);
+            if (depth != x.laneDepth) {
+                for (auto e : x.laneEvents) (void)hipEventDestroy(e);
+                x.laneEvents.clear();
+                x.laneDepth = depth;
+            }
+        }
         else throw std::runtime_error("unknown batch option: " + key);
     });
 }

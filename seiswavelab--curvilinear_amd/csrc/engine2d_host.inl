@@ -44,22 +44,25 @@ template <class R> struct Engine2D {
     size_t idx(int i, int k) const { return (size_t)off + i + (size_t)ld * k; }
     size_t internal_of(int64_t ref) const { return idx((int)(ref % nx), (int)(ref / nx)); }
 
-    // reference layout (i + nx*k, `comps` values per node) -> engine layout, converted to R
+    // reference layout (i + nx*k, `comps` values per node) -> engine layout, converted to R; written straight
+    // into page-locked staging and uploaded on the set-up stream (completed by setup_finish at construction end)
     template <class T> const T *put_layout(int slot, const double *host, int comps) {
         if (!host) return nullptr;
-        std::vector<T> v((size_t)comps * count, T(0));
+        const size_t n = (size_t)comps * count;
+        T *v = static_cast<T *>(pinned_arena().take(n * sizeof(T)));
+        std::fill(v, v + n, T(0));
         for (int k = 0; k < nz; ++k)
             for (int i = 0; i < nx; ++i)
                 for (int c = 0; c < comps; ++c)
                     v[(size_t)comps * idx(i, k) + c] = (T)host[(size_t)comps * ((size_t)i + (size_t)nx * k) + c];
-        mesh_arrays[slot].upload(v.data(), v.size() * sizeof(T));
+        mesh_arrays[slot].upload_staged(v, n * sizeof(T));
         return mesh_arrays[slot].as<T>();
     }
     template <class T> const T *put_plain(int slot, const double *host, size_t n) {
         if (!host) return nullptr;
-        std::vector<T> v(n);
+        T *v = static_cast<T *>(pinned_arena().take(std::max<size_t>(n, 1) * sizeof(T)));
         for (size_t i = 0; i < n; ++i) v[i] = (T)host[i];
-        mesh_arrays[slot].upload(v.data(), n * sizeof(T));
+        mesh_arrays[slot].upload_staged(v, n * sizeof(T));
         return mesh_arrays[slot].as<T>();
     }
 
@@ -154,11 +157,11 @@ template <class R> struct Engine2D {
         setup_metric(d);
         setup_rows(d);
         setup_regions(d);
-        for (auto &b : state) b.allocate((size_t)fields * count * sizeof(R));
-        accbuf.allocate((size_t)fields * count * sizeof(R));
+        for (auto &b : state) b.allocate_async((size_t)fields * count * sizeof(R));
+        accbuf.allocate_async((size_t)fields * count * sizeof(R));
         if (cpml) {
-            for (auto &b : pstate) b.allocate((size_t)cpml * sizeof(R));
-            paccbuf.allocate((size_t)cpml * sizeof(R));
+            for (auto &b : pstate) b.allocate_async((size_t)cpml * sizeof(R));
+            paccbuf.allocate_async((size_t)cpml * sizeof(R));
         }
         host_partials.resize(ntiles);
         std::vector<long long> pi(2 * (size_t)std::max(n, 1));
@@ -166,12 +169,15 @@ template <class R> struct Engine2D {
             require(rec[r] >= 0 && rec[r] < (int64_t)nx * nz, "receiver index outside storage");
             pi[2 * r] = (long long)internal_of(rec[r]), pi[2 * r + 1] = (long long)count + pi[2 * r];
         }
-        probes.upload(pi.data(), pi.size() * sizeof(long long));
-        status.allocate(sizeof(int));
-        accepted.allocate(sizeof(int));
-        peak.allocate(sizeof(double));
-        gather.allocate((size_t)std::max(n, 1) * 5 * sizeof(double));
-        display.allocate((size_t)(nx - 6) * (nz - 6) * sizeof(float));
+        probes.upload_async(pi.data(), pi.size() * sizeof(long long));
+        status.allocate_async(sizeof(int));
+        accepted.allocate_async(sizeof(int));
+        peak.allocate_async(sizeof(double));
+        gather.allocate_async((size_t)std::max(n, 1) * 5 * sizeof(double));
+        display.allocate_async((size_t)(nx - 6) * (nz - 6) * sizeof(float));
+        // one wait for every upload and clear above, on this thread's set-up stream only: a session built
+        // while other sessions run does not queue behind their work
+        setup_finish();
         hipFuncAttributes attr{};
         if (hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&stage_kernel<R, 0, 0, 0>)) == hipSuccess) {
             registers = attr.numRegs;
@@ -221,7 +227,7 @@ template <class R> struct Engine2D {
         std::vector<R> table(arrays.size() * (size_t)nz);
         for (size_t a = 0; a < arrays.size(); ++a)
             for (int k = 0; k < nz; ++k) table[a * nz + k] = (R)arrays[a][(size_t)nx * k];
-        rowTables.upload(table.data(), table.size() * sizeof(R));
+        rowTables.upload_async(table.data(), table.size() * sizeof(R));
         const R *base = rowTables.as<R>();
         m.lam_k = base, m.mu_k = base + nz, m.rho_k = base + 2 * nz;
         for (int l = 0; l < mech; ++l) m.dP_k[l] = base + (3 + 2 * l) * (size_t)nz, m.dS_k[l] = base + (4 + 2 * l) * (size_t)nz;
@@ -291,13 +297,13 @@ template <class R> struct Engine2D {
         }
         nBorder = (int)border.size();
         nGeneral = (nBorder + THREADS - 1) / THREADS;
-        borderNodes.upload(border.data(), std::max<size_t>(border.size(), 1) * sizeof(int2));
+        borderNodes.upload_async(border.empty() ? nullptr : border.data(), std::max<size_t>(border.size(), 1) * sizeof(int2));
         rtx = i1 > i0 ? (i1 - i0 + 64 * NV - 1) / (64 * NV) : 0;
         nInterior = rtx ? rtx * ((k1 - k0 + 3) / 4) : 0;
         nPartials = nGeneral + nInterior;
-        partials.allocate((size_t)std::max(nPartials, 1) * sizeof(Partial));
+        partials.allocate_async((size_t)std::max(nPartials, 1) * sizeof(Partial));
         nLevel2 = std::max(1, std::min(nPartials, 128));
-        level2.allocate((size_t)nLevel2 * sizeof(Partial));
+        level2.allocate_async((size_t)nLevel2 * sizeof(Partial));
     }
 
     // slot = RK stage 0..3; with profiling on, one event pair brackets the four back-to-back stage launches

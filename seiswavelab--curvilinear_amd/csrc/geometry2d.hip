@@ -65,8 +65,9 @@ __global__ __launch_bounds__(GB) void geometry2d_kernel(Geo2 g, unsigned long lo
 // nonpositive or nonfinite Jacobian, or -1 (the host raises the reference's message when it reaches it).
 long long buildGeometry2DOnDevice(Model2D &m, int device) {
     DeviceScope scope(device);
+    const hipStream_t st = setup_stream(); // set-up work waits for this thread's own stream only
     const size_t N = m.count;
-    DeviceBuffer coords, a, b, jac, depth, slope, fade, fadeDer, fail, parts;
+    SetupBuffer coords, a, b, jac, depth, slope, fade, fadeDer, fail, parts;
     coords.allocate(2 * N * sizeof(double)), a.allocate(N * sizeof(double)), b.allocate(N * sizeof(double));
     jac.allocate(N * sizeof(double));
     depth.upload(m.surfaceDepth.data(), m.surfaceDepth.size() * sizeof(double));
@@ -80,20 +81,22 @@ long long buildGeometry2DOnDevice(Model2D &m, int device) {
     parts.allocate((size_t)nb * sizeof(Cfl2));
     Geo2 g{m.nx, m.nz, m.offsetX, m.h, depth.as<double>(), slope.as<double>(), fade.as<double>(), fadeDer.as<double>(),
            coords.as<double>(), a.as<double>(), b.as<double>(), jac.as<double>()};
-    geometry2d_kernel<<<nb, GB>>>(g, fail.as<unsigned long long>(), parts.as<Cfl2>());
+    geometry2d_kernel<<<nb, GB, 0, st>>>(g, fail.as<unsigned long long>(), parts.as<Cfl2>());
     check(hipGetLastError(), "2D geometry launch");
     unsigned long long first = none;
-    check(hipMemcpy(&first, fail.ptr, sizeof first, hipMemcpyDeviceToHost), "2D geometry");
+    check(hipMemcpyAsync(&first, fail.ptr, sizeof first, hipMemcpyDeviceToHost, st), "2D geometry");
+    check(hipStreamSynchronize(st), "2D geometry");
     std::vector<Cfl2> h(nb);
-    check(hipMemcpy(h.data(), parts.ptr, h.size() * sizeof(Cfl2), hipMemcpyDeviceToHost), "2D CFL");
+    m.coordinates.resize(2 * N), m.a.resize(N), m.b.resize(N), m.jac.resize(N);
+    check(hipMemcpyAsync(h.data(), parts.ptr, h.size() * sizeof(Cfl2), hipMemcpyDeviceToHost, st), "2D CFL");
+    check(hipMemcpyAsync(m.coordinates.data(), coords.ptr, 2 * N * sizeof(double), hipMemcpyDeviceToHost, st), "2D geometry");
+    check(hipMemcpyAsync(m.a.data(), a.ptr, N * sizeof(double), hipMemcpyDeviceToHost, st), "2D geometry");
+    check(hipMemcpyAsync(m.b.data(), b.ptr, N * sizeof(double), hipMemcpyDeviceToHost, st), "2D geometry");
+    check(hipMemcpyAsync(m.jac.data(), jac.ptr, N * sizeof(double), hipMemcpyDeviceToHost, st), "2D geometry");
+    check(hipStreamSynchronize(st), "2D geometry");
     Cfl2 r{DBL_MAX, 0.0};
     for (const auto &x : h) r.minJ = std::min(r.minJ, x.minJ), r.maxMetric = std::max(r.maxMetric, x.maxMetric);
     m.minJacobian = r.minJ, m.maxMetric = r.maxMetric;
-    m.coordinates.resize(2 * N), m.a.resize(N), m.b.resize(N), m.jac.resize(N);
-    check(hipMemcpy(m.coordinates.data(), coords.ptr, 2 * N * sizeof(double), hipMemcpyDeviceToHost), "2D geometry");
-    check(hipMemcpy(m.a.data(), a.ptr, N * sizeof(double), hipMemcpyDeviceToHost), "2D geometry");
-    check(hipMemcpy(m.b.data(), b.ptr, N * sizeof(double), hipMemcpyDeviceToHost), "2D geometry");
-    check(hipMemcpy(m.jac.data(), jac.ptr, N * sizeof(double), hipMemcpyDeviceToHost), "2D geometry");
     return first == none ? -1 : (long long)first;
 }
 

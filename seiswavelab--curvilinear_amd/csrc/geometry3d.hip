@@ -150,8 +150,9 @@ __global__ __launch_bounds__(GB) void cfl_kernel(Geo g, Cfl *partials) {
 // m.maxEdge on `device` (the values the host loops in model3d.cpp produce, bit for bit).
 void buildGeometry3DOnDevice(Model3D &m, const std::vector<double> &surface, const std::vector<double> &fade, int device) {
     DeviceScope scope(device);
+    const hipStream_t st = setup_stream(); // set-up work waits for this thread's own stream only
     const size_t N = m.count;
-    DeviceBuffer coords, metrics, jac, normals, dsurf, dfade, fail, parts;
+    SetupBuffer coords, metrics, jac, normals, dsurf, dfade, fail, parts;
     coords.allocate(3 * N * sizeof(double)), metrics.allocate(9 * N * sizeof(double));
     jac.allocate(N * sizeof(double)), normals.allocate(3 * N * sizeof(double));
     dsurf.upload(surface.data(), std::max<size_t>(surface.size(), 1) * sizeof(double));
@@ -162,20 +163,27 @@ void buildGeometry3DOnDevice(Model3D &m, const std::vector<double> &surface, con
           normals.as<double>()};
     const size_t interior = (size_t)(m.nx - 2 * G3) * (m.ny - 2 * G3) * (m.nz - 2 * G3);
     const unsigned nb = (unsigned)((interior + GB - 1) / GB);
-    coords_kernel<<<(unsigned)((N + GB - 1) / GB), GB>>>(g, dsurf.as<double>(), dfade.as<double>());
+    coords_kernel<<<(unsigned)((N + GB - 1) / GB), GB, 0, st>>>(g, dsurf.as<double>(), dfade.as<double>());
     check(hipGetLastError(), "3D geometry launch");
-    metrics_kernel<<<nb, GB>>>(g, fail.as<unsigned long long>());
+    metrics_kernel<<<nb, GB, 0, st>>>(g, fail.as<unsigned long long>());
     check(hipGetLastError(), "3D geometry launch");
     unsigned long long first = none;
-    check(hipMemcpy(&first, fail.ptr, sizeof first, hipMemcpyDeviceToHost), "3D geometry");
+    check(hipMemcpyAsync(&first, fail.ptr, sizeof first, hipMemcpyDeviceToHost, st), "3D geometry");
+    check(hipStreamSynchronize(st), "3D geometry");
     if (first != none)
         throw std::runtime_error(first & 1 ? "Could not compute topographic surface normal."
                                            : "Curvilinear mapping has a non-positive or non-finite Jacobian.");
     parts.allocate((size_t)nb * sizeof(Cfl));
-    cfl_kernel<<<nb, GB>>>(g, parts.as<Cfl>());
+    cfl_kernel<<<nb, GB, 0, st>>>(g, parts.as<Cfl>());
     check(hipGetLastError(), "3D CFL launch");
     std::vector<Cfl> h(nb);
-    check(hipMemcpy(h.data(), parts.ptr, h.size() * sizeof(Cfl), hipMemcpyDeviceToHost), "3D CFL");
+    check(hipMemcpyAsync(h.data(), parts.ptr, h.size() * sizeof(Cfl), hipMemcpyDeviceToHost, st), "3D CFL");
+    m.coordinates.resize(3 * N), m.metrics.resize(9 * N), m.jacobian.resize(N), m.normals.resize(3 * N);
+    check(hipMemcpyAsync(m.coordinates.data(), coords.ptr, 3 * N * sizeof(double), hipMemcpyDeviceToHost, st), "3D geometry");
+    check(hipMemcpyAsync(m.metrics.data(), metrics.ptr, 9 * N * sizeof(double), hipMemcpyDeviceToHost, st), "3D geometry");
+    check(hipMemcpyAsync(m.jacobian.data(), jac.ptr, N * sizeof(double), hipMemcpyDeviceToHost, st), "3D geometry");
+    check(hipMemcpyAsync(m.normals.data(), normals.ptr, 3 * N * sizeof(double), hipMemcpyDeviceToHost, st), "3D geometry");
+    check(hipStreamSynchronize(st), "3D geometry");
     Cfl r{DBL_MAX, 0.0, DBL_MAX, 0.0, 0, 0};
     for (const auto &x : h) {
         r.minJ = std::min(r.minJ, x.minJ), r.maxNorm = std::max(r.maxNorm, x.maxNorm);
@@ -186,11 +194,6 @@ void buildGeometry3DOnDevice(Model3D &m, const std::vector<double> &surface, con
     m.minJacobian = r.minJ, m.maxMetricNorm = r.maxNorm;
     m.minLength = m.terrain ? r.minLength : m.p.spacing / std::sqrt(3.0);
     m.maxEdge = m.terrain ? r.maxEdge : m.p.spacing;
-    m.coordinates.resize(3 * N), m.metrics.resize(9 * N), m.jacobian.resize(N), m.normals.resize(3 * N);
-    check(hipMemcpy(m.coordinates.data(), coords.ptr, 3 * N * sizeof(double), hipMemcpyDeviceToHost), "3D geometry");
-    check(hipMemcpy(m.metrics.data(), metrics.ptr, 9 * N * sizeof(double), hipMemcpyDeviceToHost), "3D geometry");
-    check(hipMemcpy(m.jacobian.data(), jac.ptr, N * sizeof(double), hipMemcpyDeviceToHost), "3D geometry");
-    check(hipMemcpy(m.normals.data(), normals.ptr, 3 * N * sizeof(double), hipMemcpyDeviceToHost), "3D geometry");
 }
 
 } // namespace swl

@@ -76,13 +76,13 @@ std::array<double, 3> nnlsStrengths(double q, const GslsFit &f, double fmin, dou
     need(std::isfinite(bestCost), "No passive GSLS fit for this Q/frequency band.");
     return best;
 }
-double relaxedModulus(const GslsFit &f, const std::array<double, 3> &weights, double fref, double rho, double v) {
+double velocityScale(const GslsFit &f, const std::array<double, 3> &weights, double fref) {
     const std::complex<double> w{0, 2 * kPi * fref};
     std::complex<double> shape = 1.;
     for (int l = 0; l < f.mechanisms; ++l) shape += weights[l] * w / (w + f.rates[l]);
-    const double inv = std::real(1. / std::sqrt(shape));
-    return rho * v * v * inv * inv;
+    return std::real(1. / std::sqrt(shape));
 }
+double relaxedModulus(double inv, double rho, double v) { return rho * v * v * inv * inv; }
 double qualityFactor(const GslsFit &f, const GslsMaterial &m, bool shear, double freq) {
     const std::complex<double> s{0, 2 * kPi * freq};
     std::complex<double> c = shear ? m.sRelaxed : m.pRelaxed;
@@ -104,7 +104,7 @@ double lerpExact(double a, double b, double t) {
     return (t > 1) == (b > a) ? (b < x ? x : b) : (b > x ? x : b);
 }
 
-GslsFit gslsFit(int mechanisms, double fmin, double fmax, double qp, double qs) {
+GslsFit gslsFit(int mechanisms, double fmin, double fmax, double qp, double qs, double fref) {
     GslsFit f;
     f.mechanisms = mechanisms;
     for (int l = 0; l < mechanisms; ++l)
@@ -112,14 +112,16 @@ GslsFit gslsFit(int mechanisms, double fmin, double fmax, double qp, double qs) 
                      (mechanisms == 1 ? std::sqrt(fmin * fmax) : fmin * std::pow(fmax / fmin, double(l) / (mechanisms - 1)));
     f.strengthP = nnlsStrengths(qp, f, fmin, fmax);
     f.strengthS = nnlsStrengths(qs, f, fmin, fmax);
+    f.scaleFref = fref, f.invP = velocityScale(f, f.strengthP, fref), f.invS = velocityScale(f, f.strengthS, fref);
     return f;
 }
 
 GslsMaterial gslsMaterial(const GslsFit &f, double fref, double fmin, double fmax, double vp, double vs, double rho,
                           double qp, double qs, bool qError) {
     GslsMaterial m;
-    m.pRelaxed = relaxedModulus(f, f.strengthP, fref, rho, vp);
-    m.sRelaxed = relaxedModulus(f, f.strengthS, fref, rho, vs);
+    const bool cached = f.scaleFref == fref;
+    m.pRelaxed = relaxedModulus(cached ? f.invP : velocityScale(f, f.strengthP, fref), rho, vp);
+    m.sRelaxed = relaxedModulus(cached ? f.invS : velocityScale(f, f.strengthS, fref), rho, vs);
     m.pUnrelaxed = m.pRelaxed, m.sUnrelaxed = m.sRelaxed;
     need(m.pRelaxed > 4 * m.sRelaxed / 3, "Qp/Qs produce nonpositive relaxed bulk modulus.");
     for (int l = 0; l < f.mechanisms; ++l) {
@@ -268,7 +270,7 @@ Model2D buildModel2D(const swl_params2d &requested, int device) {
             need(std::isfinite(vp) && std::isfinite(vs) && std::isfinite(rho) && rho > 0 && vs > 0 && vp * vp > 4 * vs * vs / 3,
                  "Invalid GSLS reference velocities/density.");
             for (double q : {qp, qs}) need(std::isfinite(q) && q >= 5 && q <= 1e9, "Layer Qp/Qs must be in 5..1e9.");
-            fits.push_back(gslsFit(p.mechanisms, p.q_min_frequency, p.q_max_frequency, qp, qs));
+            fits.push_back(gslsFit(p.mechanisms, p.q_min_frequency, p.q_max_frequency, qp, qs, p.q_reference_frequency));
             rheology.push_back(gslsMaterial(fits.back(), p.q_reference_frequency, p.q_min_frequency, p.q_max_frequency, vp,
                                             vs, rho, qp, qs, true));
         }

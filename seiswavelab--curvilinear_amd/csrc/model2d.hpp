@@ -14,12 +14,15 @@ namespace swl {
 struct GslsFit { // SeisWave::Rheology fit restricted to one (Qp, Qs) pair, src/common/Gsls.cpp:11-78
     int mechanisms = 0;
     std::array<double, 3> rates{}, strengthP{}, strengthS{};
+    // Re(1/sqrt(shape)) of the P and S strengths at `scaleFref` (Gsls.cpp:45): a property of the fit alone, so
+    // per-node materials (relaxed modulus = rho v^2 inv^2) reuse it instead of repeating the complex algebra
+    double scaleFref = -1, invP = 0, invS = 0;
 };
 struct GslsMaterial {
     double pRelaxed = 0, sRelaxed = 0, pUnrelaxed = 0, sUnrelaxed = 0, maxRelativeQError = 0;
     std::array<double, 3> deltaP{}, deltaS{};
 };
-GslsFit gslsFit(int mechanisms, double fmin, double fmax, double qp, double qs);
+GslsFit gslsFit(int mechanisms, double fmin, double fmax, double qp, double qs, double fref);
 GslsMaterial gslsMaterial(const GslsFit &fit, double fref, double fmin, double fmax, double vp, double vs, double rho,
                           double qp, double qs, bool qError);
 

@@ -62,13 +62,31 @@ def test_invalid_parameters_rejected_with_reference_messages():
     assert not ok and "without a device" in err
 
 
-PLAN_CASES = sorted(REF_CASES) + ["elastic_flat", "elastic_curved_moment", "gsls3_curved_pml", "dry_loess_gsls3_pml"]
+PLAN_CASES = sorted(REF_CASES) + ["elastic_flat", "elastic_curved_moment", "gsls3_curved_pml", "dry_loess_gsls3_pml",
+                                   "per_node_gsls2_pml"]
+
+
+def plan_case(name):
+    if name == "per_node_gsls2_pml":
+        # per-node velocities and density (seeded 5 % perturbation) on the layered GSLS-2 model: every node gets
+        # its own relaxed and unrelaxed moduli from its layer's fit
+        p = ref_case("gsls2_layered_pml")
+        rng = np.random.default_rng(11)
+        depth = np.arange(p.nz)[:, None] * p.spacing * np.ones((1, p.nx))
+        bounds = np.cumsum([l.thickness for l in p.layers])
+        idx = np.minimum(np.searchsorted(bounds, depth, side="right"), len(p.layers) - 1)
+        base = {k: np.array([getattr(l, k) for l in p.layers])[idx] for k in ("vp", "vs", "density")}
+        scale = 1 + 0.05 * rng.standard_normal(depth.shape)  # one field for both speeds keeps Vp/Vs per layer
+        p.nodeVp, p.nodeVs = (base["vp"] * scale).ravel(), (base["vs"] * scale).ravel()
+        p.nodeDensity = (base["density"] * (1 + 0.02 * rng.standard_normal(depth.shape))).ravel()
+        return p
+    return ref_case(name) if name in REF_CASES else precision_case(name)
 
 
 @pytest.mark.parametrize("name", PLAN_CASES)
 @pytest.mark.parametrize("prec", [Precision.Float64, Precision.Float32])
 def test_planned_session_bitwise_equals_oracle(name, prec):
-    p = ref_case(name) if name in REF_CASES else precision_case(name)
+    p = plan_case(name)
     p.precision = prec
     s = Simulation()
     ok, err = s.initialize(p, BackendKind.Cpu)
