@@ -315,7 +315,12 @@ __global__ __launch_bounds__(THREADS) void stage3_kernel(Stage3<R> A) {
     __shared__ Partial3 sh[THREADS];
     if (*A.status) return;
     const DMesh3<R> &m = A.m;
-    const int t = A.t0 + tile_of_block(blockIdx.x, A.nband);
+    // Each XCD takes a contiguous run of tile columns, planes varying fastest and then the y tile: the z- and
+    // y-neighbours of a block's tile (the 3 + 1 halo planes and rows) are tiles its XCD's other resident blocks work
+    // on at the same time (L2 hits), instead of tiles a plane or a row of tiles later.  t is still the tile's
+    // plane-major index, so partials keep their order.
+    const int tilesPerPlane = A.ntx * A.nty, u = tile_of_block(blockIdx.x, A.nband), planes = A.nband / tilesPerPlane;
+    const int t = A.t0 + (u % planes) * tilesPerPlane + u / planes;
     if constexpr (BAND != SURFACE) {
         const int per = A.ntx * A.nty, kk = t / per, rem = t % per;
         const int i = 3 + (rem % A.ntx) * TX + (threadIdx.x % TX), j = 3 + (rem / A.ntx) * TY + threadIdx.x / TX, k = 3 + kk;
