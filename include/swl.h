@@ -452,6 +452,18 @@ int swl_sim3d_model(const swl_sim3d *s, int which, double *out);
 int swl_sim3d_read_display_frame(const swl_sim3d *s, int x, int y, int z, int component, float *out);
 int swl_sim3d_profile(swl_sim3d *s, int enable);
 int swl_sim3d_kernel_timing(const swl_sim3d *s, double *out);
+/* writeReceiversCsv(simulation, path, components, error) (include/Seis3D/ReceiverIo.hpp:5-7,
+   src/three_d/ReceiverIo.cpp:9-51): components[3] = Vx, Vy, Vz; one row per accepted step, t = 0 included. */
+int swl_sim3d_write_receivers_csv(const swl_sim3d *s, const char *path, const int32_t *components);
+/* writeReceiversCsv(simulation, path, error, allVelocityComponents) (ReceiverIo.hpp:8-9): Vz only with the legacy
+   column names, or all three components. */
+int swl_sim3d_write_receivers_vz_csv(const swl_sim3d *s, const char *path, int all_velocity_components);
+/* writeVts(snapshot(), path, VtsExportOptions3D, error) (include/Seis3D/VtsWriter.hpp:11-24,
+   src/three_d/VtsWriter.cpp:17-117): fields[11] = Vx, Vy, Vz, |V|, mean normal stress, Sxx, Syy, Szz, Sxy, Sxz,
+   Syz; materials (per node with layer ids for layered models); the velocity vector; written to `path`.tmp-<ticks>
+   and renamed. */
+int swl_sim3d_write_vts(const swl_sim3d *s, const char *path, const int32_t *fields, int include_material,
+                        int include_velocity_vector);
 
 #ifdef __cplusplus
 }

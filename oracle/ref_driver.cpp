@@ -394,3 +394,33 @@ int ref2d_write_snapshot(void *h, const char *path, const int *components, int i
     return ok;
 }
 } // extern "C"
+
+// ---- 3D receiver CSV and wavefield VTS: the reference's own writers (include/Seis3D/ReceiverIo.hpp,
+//      include/Seis3D/VtsWriter.hpp) on a reference CPU session
+#include "Seis3D/ReceiverIo.hpp"
+#include "Seis3D/VtsWriter.hpp"
+extern "C" {
+int ref3d_write_receivers_csv(void *h, const char *path, const int *components, char *error, int len) {
+    std::string e;
+    const std::array<bool, 3> sel{components[0] != 0, components[1] != 0, components[2] != 0};
+    const bool ok = S3::writeReceiversCsv(*static_cast<S3::CpuSimulation3D *>(h), path, sel, e);
+    copyError(e, error, len);
+    return ok;
+}
+int ref3d_write_receivers_vz_csv(void *h, const char *path, int all, char *error, int len) {
+    std::string e;
+    const bool ok = S3::writeReceiversCsv(*static_cast<S3::CpuSimulation3D *>(h), path, e, all != 0);
+    copyError(e, error, len);
+    return ok;
+}
+int ref3d_write_vts(void *h, const char *path, const int *fields, int include_material, int include_vector, char *error,
+                    int len) {
+    S3::VtsExportOptions3D o;
+    for (int n = 0; n < 11; ++n) o.fields[n] = fields[n] != 0;
+    o.includeMaterial = include_material != 0, o.includeVelocityVector = include_vector != 0;
+    std::string e;
+    const bool ok = S3::writeVts(static_cast<S3::CpuSimulation3D *>(h)->snapshot(), path, o, e);
+    copyError(e, error, len);
+    return ok;
+}
+} // extern "C"

@@ -10,5 +10,5 @@ from .simulation import (AsyncSimulation, Batch2D, Receiver, ReceiverTrace3D, Si
                          writeReceivers)
 from . import modelio, native, records, workloads
 from .records import ImportedRecords, SegyExportProfile, TraceRecord, read_segy, write_segy
-from .modelio import (ComputeSettings, ModelDocument, SnapshotExportOptions, displayedSnapshotOptions, readModelVts,
-                      writeModelVts, writeSnapshot)
+from .modelio import (ComputeSettings, ModelDocument, SnapshotExportOptions, VtsExportOptions3D,
+                      displayedSnapshotOptions, readModelVts, writeModelVts, writeReceiversCsv, writeSnapshot, writeVts)

@@ -300,6 +300,9 @@ void cflGeometry(Model3D &m) {
 
 } // namespace
 
+// layer of node q as the material pass assigns it (MaterialModels3D.hpp:19-31)
+int layerOfNode3D(const Model3D &m, size_t q) { return layerAt(m.p, materialDepth(m, q)); }
+
 Model3D buildModel3D(const swl_params3d &requested, int device) {
     validate(requested);
     Model3D m;

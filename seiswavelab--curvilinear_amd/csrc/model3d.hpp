@@ -48,5 +48,6 @@ Model3D buildModel3D(const swl_params3d &requested, int device = -1);
 // displacement and the per-plane fade.
 void buildGeometry3DOnDevice(Model3D &m, const std::vector<double> &surface, const std::vector<double> &fade, int device);
 swl_mesh3d_desc meshDescriptor3D(const Model3D &m, int precision);
+int layerOfNode3D(const Model3D &m, size_t q); // layered models: the node's layer index
 
 } // namespace swl

@@ -91,3 +91,39 @@ def writeSnapshot(sim, path, options: Optional[SnapshotExportOptions] = None) ->
     comps = (C.c_int32 * 7)(*[int(bool(v)) for v in options.components])
     rc = native.lib().swl_sim2d_write_snapshot(_handle(sim), str(path).encode(), comps, int(bool(options.includeMaterial)))
     return (rc == 0), ("" if rc == 0 else native.last_error())
+
+
+# ---------------------------------------------------------------------------------------------------------
+# 3D: receiver CSV (include/Seis3D/ReceiverIo.hpp) and wavefield VTS (include/Seis3D/VtsWriter.hpp)
+
+@dataclass
+class VtsExportOptions3D:
+    """Vx, Vy, Vz, |V|, mean normal stress, Sxx, Syy, Szz, Sxy, Sxz, Syz; materials; the velocity vector."""
+    fields: List[bool] = field(default_factory=lambda: [True] * 11)
+    includeMaterial: bool = True
+    includeVelocityVector: bool = True
+
+
+def writeReceiversCsv(sim, path, components=None, allVelocityComponents: bool = False) -> Tuple[bool, str]:
+    """Both overloads of the reference: `components` = (Vx, Vy, Vz) flags, or (without it) Vz with the legacy
+    column names, or all three with `allVelocityComponents`."""
+    lib = native.lib()
+    if components is not None:
+        if len(components) != 3:
+            return False, "Receiver export needs three component flags."
+        sel = (C.c_int32 * 3)(*[int(bool(v)) for v in components])
+        rc = lib.swl_sim3d_write_receivers_csv(_handle(sim), str(path).encode(), sel)
+    else:
+        rc = lib.swl_sim3d_write_receivers_vz_csv(_handle(sim), str(path).encode(), int(bool(allVelocityComponents)))
+    return (rc == 0), ("" if rc == 0 else native.last_error())
+
+
+def writeVts(sim, path, options: Optional[VtsExportOptions3D] = None) -> Tuple[bool, str]:
+    """Snapshot of the current 3D state as a VTK StructuredGrid of the physical nodes."""
+    options = options or VtsExportOptions3D()
+    if len(options.fields) != 11:
+        return False, "VTS export needs eleven field flags."
+    fields = (C.c_int32 * 11)(*[int(bool(v)) for v in options.fields])
+    rc = native.lib().swl_sim3d_write_vts(_handle(sim), str(path).encode(), fields, int(bool(options.includeMaterial)),
+                                          int(bool(options.includeVelocityVector)))
+    return (rc == 0), ("" if rc == 0 else native.last_error())

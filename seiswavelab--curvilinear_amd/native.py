@@ -87,6 +87,9 @@ SIGNATURES = {
     "swl_model_doc2d_params": (C.c_int, [vp, vp]),
     "swl_model_doc2d_info": (C.c_int, [vp, vp]),
     "swl_sim2d_write_snapshot": (C.c_int, [vp, C.c_char_p, vp, C.c_int]),
+    "swl_sim3d_write_receivers_csv": (C.c_int, [vp, C.c_char_p, vp]),
+    "swl_sim3d_write_receivers_vz_csv": (C.c_int, [vp, C.c_char_p, C.c_int]),
+    "swl_sim3d_write_vts": (C.c_int, [vp, C.c_char_p, vp, C.c_int, C.c_int]),
     "swl_async2d_create": (C.c_int, [P(vp)]),
     "swl_async2d_destroy": (None, [vp]),
     "swl_async2d_initialize": (C.c_int, [vp, vp, C.c_int, C.c_int]),

@@ -6,7 +6,8 @@ tests/test_oracle_2d.py (REF_CASES) and tests/test_oracle_3d.py (CASES) and keep
 dt, total steps, the energy and peak energy after every step, the receiver traces, and per final
 field its L2 norm, sum and max |value|.  The inputs are the case definitions in those test files.
 It also runs the reference's SEG-Y writer on a synthetic record set, and its model-document and
-wavefield-snapshot writers on the small VTS cases of tests/reference_fixtures.py (gzipped text).
+wavefield-snapshot writers on the small VTS cases of tests/reference_fixtures.py (gzipped text), and its
+3D VTS and receiver-CSV writers on VTS3D_CASES.
 
     python tests/golden/make_reference_fixtures.py      (needs oracle/_ref/libswlref.so)
 """
@@ -19,9 +20,10 @@ HERE = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(HERE.parent))
 import conftest  # noqa: E402,F401  registers the package
 from cpu_sessions import load  # noqa: E402
-from reference_fixtures import (FIXTURES_2D, SEGY_VARIANTS, VTS_ALL_COMPONENTS, VTS_MODEL_VARIANTS,  # noqa: E402
-                                VTS_SNAPSHOT_CASES, fixture_path, run_oracle_2d, run_oracle_3d, segy_fixture_path,
-                                segy_records, vts_case, vts_fixture_path)
+from reference_fixtures import (FIXTURES_2D, RECEIVERS3D_EXPORTS, SEGY_VARIANTS, VTS3D_CASES,  # noqa: E402
+                                VTS_ALL_COMPONENTS, VTS_MODEL_VARIANTS, VTS_SNAPSHOT_CASES, fixture_path,
+                                receivers3d_fixture_path, run_oracle_2d, run_oracle_3d, segy_fixture_path,
+                                segy_records, vts3d_fixture_path, vts_case, vts_fixture_path)
 from test_oracle_2d import ref_case  # noqa: E402
 from test_oracle_3d import CASES, case  # noqa: E402
 
@@ -73,6 +75,41 @@ def write_vts_fixtures():
                 keep(f"{tmp}/s.vts", vts_fixture_path(name, step))
 
 
+def write_export3d_fixtures():
+    """The reference's 3D writeVts (all fields, materials, velocity vector) at step 0 and its writeReceiversCsv
+    after maxSteps, on reference CPU sessions of VTS3D_CASES."""
+    import ctypes as C
+    import gzip
+    import shutil
+    import tempfile
+    from cpu_sessions import CpuSession3D
+    lib = load("ref")
+    lib.ref3d_write_vts.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_int, C.c_int, C.c_char_p, C.c_int]
+    lib.ref3d_write_receivers_csv.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_char_p, C.c_int]
+    lib.ref3d_write_receivers_vz_csv.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_char_p, C.c_int]
+    err = C.create_string_buffer(512)
+    with tempfile.TemporaryDirectory() as tmp:
+        def keep(src, dst):
+            with open(src, "rb") as a, gzip.GzipFile(dst, "wb", mtime=0) as b:
+                shutil.copyfileobj(a, b)
+        for name in VTS3D_CASES:
+            p = case(name)
+            s = CpuSession3D("ref", p)
+            assert lib.ref3d_write_vts(C.c_void_p(s.h), f"{tmp}/s.vts".encode(), (C.c_int * 11)(*[1] * 11), 1, 1,
+                                       err, 512), err.value
+            keep(f"{tmp}/s.vts", vts3d_fixture_path(name))
+            ok, e = s.advance(p.maxSteps)
+            assert ok, e
+            if RECEIVERS3D_EXPORTS[name] == "vz":
+                ok = lib.ref3d_write_receivers_vz_csv(C.c_void_p(s.h), f"{tmp}/r.csv".encode(), 0, err, 512)
+            else:
+                ok = lib.ref3d_write_receivers_csv(C.c_void_p(s.h), f"{tmp}/r.csv".encode(), (C.c_int * 3)(1, 0, 1),
+                                                   err, 512)
+            assert ok, err.value
+            keep(f"{tmp}/r.csv", receivers3d_fixture_path(name))
+            s.close()
+
+
 def main():
     if load("ref") is None:
         sys.exit("oracle/_ref/libswlref.so is missing: build it with `make -C oracle ref` first")
@@ -84,7 +121,8 @@ def main():
         np.savez_compressed(fixture_path("3d", name), **run_oracle_3d("ref", case(name)))
     write_segy_fixtures()
     write_vts_fixtures()
-    for f in sorted(list(HERE.glob("reference_*.npz")) + list(HERE.glob("segy_*.sgy")) + list(HERE.glob("vts_*.gz"))):
+    write_export3d_fixtures()
+    for f in sorted(list(HERE.glob("reference_*.npz")) + list(HERE.glob("segy_*.sgy")) + list(HERE.glob("*.gz"))):
         print(f.name, f.stat().st_size)
 
 

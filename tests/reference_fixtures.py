@@ -132,6 +132,21 @@ def vts_fixture_path(name, step=None):
     return GOLDEN / (f"vts_model_{name}.vts.gz" if step is None else f"vts_snapshot_{name}_step{step}.vts.gz")
 
 
+# ---- 3D receiver CSV and wavefield VTS (tests/golden/vts3d_*, receivers3d_*: the reference's own writers) ----
+VTS3D_CASES = ["analytic_sponge_force", "dem_layers_moment"]  # cases of tests/test_oracle_3d.py
+RECEIVERS3D_EXPORTS = {"analytic_sponge_force": "vz", "dem_layers_moment": "x_z"}  # legacy Vz / Vx+Vz columns
+
+
+def vts3d_fixture_path(name):
+    """writeVts of the step-0 state: all eleven fields, the materials and the velocity vector."""
+    return GOLDEN / f"vts3d_{name}_step0.vts.gz"
+
+
+def receivers3d_fixture_path(name):
+    """writeReceiversCsv after the case's maxSteps, in the RECEIVERS3D_EXPORTS form."""
+    return GOLDEN / f"receivers3d_{name}_{RECEIVERS3D_EXPORTS[name]}.csv.gz"
+
+
 def vts_text(path):
     import gzip
     opener = gzip.open if str(path).endswith(".gz") else open
@@ -140,8 +155,9 @@ def vts_text(path):
 
 
 def parse_vts(text):
-    """Independent reading of a VTS file of either schema: {"field": {name: [values]}, "point": {name: array},
-    "coordinates": (n, 3) array, "order": [field array names in file order]}."""
+    """Independent reading of a VTS file (2D documents and snapshots, 3D snapshots): {"field": {name: [values]},
+    "point": {name: array; the 3D velocity vector as (n, 3)}, "coordinates": (n, 3) array, "order": [field array
+    names in file order]}."""
     import re
     out = {"field": {}, "point": {}, "coordinates": None, "order": []}
     field_end = text.index("</FieldData>")
@@ -151,10 +167,10 @@ def parse_vts(text):
         if m.start() < field_end:
             out["field"][attrs["Name"]] = values
             out["order"].append(attrs["Name"])
-        elif attrs.get("NumberOfComponents") == "3":
+        elif attrs.get("NumberOfComponents") == "3" and attrs.get("Name") != "Velocity":
             out["coordinates"] = values.reshape(-1, 3)
         else:
-            out["point"][attrs["Name"]] = values
+            out["point"][attrs["Name"]] = values.reshape(-1, 3) if attrs.get("NumberOfComponents") == "3" else values
     return out
 
 
