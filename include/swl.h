@@ -261,7 +261,7 @@ typedef struct swl_batch_shot {
 typedef struct swl_batch2d_runner swl_batch2d_runner;
 int swl_batch2d_runner_create(int device, swl_batch2d_runner **out);
 void swl_batch2d_runner_destroy(swl_batch2d_runner *r);
-/* options: "fused" (default 0 = lanes), "lanes" (2 streams), "lane_steps" (32), "lane_depth" (2 turns in flight
+/* options: "fused" (default 0 = lanes), "lanes" (3 streams), "lane_steps" (32), "lane_depth" (2 turns in flight
    per lane); fused mode: "graphs" (1),
    "autotune" (1), "grid" (0 = tuned / one block per tile), "chunk" (32 steps) */
 int swl_batch2d_runner_set_option(swl_batch2d_runner *r, const char *name, int value);

@@ -92,9 +92,10 @@ struct Runner2D {
     // fused = 1: the batch kernels above (one launch per stage over every shot's tiles);
     // fused = 0: each shot keeps its own specialised stage kernels and the shots are spread over `lanes`
     // streams (longest-first onto the least-loaded lane), so small shots' launches overlap on the device.
-    // Two lanes measured best for shots of ~0.1-1 M nodes: two shots' working sets (~100 MB each for
-    // GSLS-3 FP32) still fit the 256 MB Infinity Cache together, four do not.
-    int fused = 0, lanes = 2, laneSteps = 32;
+    // Three lanes measured best for shots of ~0.1-1 M nodes (GSLS-3 FP32 with the memory variables rebuilt
+    // per node: ~70 MB of state per 0.33 M-node shot); four lanes' working sets overflow the 256 MB
+    // Infinity Cache.
+    int fused = 0, lanes = 3, laneSteps = 32;
     std::vector<hipStream_t> laneStreams;
     std::vector<hipEvent_t> laneEvents; // laneDepth per lane: ends of that lane's last turns (a ring)
     int laneDepth = 2;                  // turns a lane may have enqueued ahead of the device

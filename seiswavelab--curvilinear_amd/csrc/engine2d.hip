@@ -85,6 +85,8 @@ template <class R> struct StageArgs {
     R *pout, *pacc;
     size_t count;
     R forcing, w, wy; // wy: weight of this stage's k in Y = U^n + dt(k1 + 2k2 + 2k3 + k4)/6 (FP32 interior)
+    R hdt, dt, dt6;   // dt / 2, dt, dt / 6 (memory variables rebuilt from U^n, FP32 interior)
+    int stage;        // 0..3
     int kind;         // 0 first stage, 1 middle stages, 2 final stage (+ inspection)
     int ystage;       // FP32 interior, stage 2: Y = U^n + dt/6 k1 is rebuilt from the input and the base
     const int *status;
@@ -532,6 +534,42 @@ __device__ __forceinline__ void combine_field(const StageArgs<R> &A, size_t o, b
     vstore(A.out + o, vo, A.nt);
     if (A.kind == 1 || (A.kind == 0 && !kFast<R>)) vstore(A.acc + o, vacc, A.nt);
 }
+// FP32 interior nodes with two or more GSLS mechanisms: a memory variable obeys the pointwise ODE
+// dm/dt = r (d - m), whose drive d is a combination of three strains (exx, ezz, exz) with the mechanism's
+// dP, dS.  Its stage inputs are therefore rebuilt from U^n and the strains of the step's earlier stages
+// instead of being streamed: each of stages 1-3 stores its three strains (stages 1 and 2 in the
+// accumulator's memory slots, stage 3 in its output's, which is stage 4's input), and stage 4 forms
+// k1..k4 and writes U^{n+1} = U^n + dt (k1 + 2 k2 + 2 k3 + k4) / 6.  Syy is pointwise too and no other
+// rate reads it (dSyy/dt = lambda div - sum of the yy memories), so it and the yy memories are formed at
+// stage 4 only.  The same RK4, rounded differently; per node and step the 13 x 14 accesses of the
+// memories and Syy become 80 (three mechanisms).  Border nodes stream theirs as before; the two sets of
+// nodes are disjoint and the scratch slots are per node.
+template <class R, int MECH> constexpr bool kRebuildMemory = kFast<R> && MECH >= 2;
+// drive of memory component f (xx, zz, xz, yy) for a mechanism with moduli dP, dS
+template <class R> __device__ __forceinline__ R memory_drive(int f, R dp, R ds, R exx, R ezz, R exz) {
+    const R dl = dp - 2 * ds, div = exx + ezz;
+    return f == 0 ? mad(dl, div, 2 * ds * exx) : f == 1 ? mad(dl, div, 2 * ds * ezz) : f == 2 ? ds * exz : dl * div;
+}
+// the stage input m_s of a memory variable after s earlier stages with drives d[0..s-1]; with s = 3 also
+// the weighted sums k1 + 2 k2 + 2 k3 of its rates and m_0 + 2 m_1 + 2 m_2 of its stage inputs
+template <class R>
+__device__ __forceinline__ R memory_stage(int s, R r, R m0, const R (&d)[3], R hdt, R dt, R &ksum, R &msum) {
+    R m = m0;
+    ksum = 0, msum = m0;
+    if (s >= 1) {
+        const R k = r * (d[0] - m);
+        ksum = k, m = mad(hdt, k, m0), msum = mad((R)2, m, msum);
+    }
+    if (s >= 2) {
+        const R k = r * (d[1] - m);
+        ksum = mad((R)2, k, ksum), m = mad(hdt, k, m0), msum = mad((R)2, m, msum);
+    }
+    if (s >= 3) {
+        const R k = r * (d[2] - m);
+        ksum = mad((R)2, k, ksum), m = mad(dt, k, m0);
+    }
+    return m;
+}
 // LDS staging of the stencil input (the `tile` variant): per field, the block's 4 rows plus the 3 + 1
 // halo rows of the z-stencil (8 rows) and HV halo vectors on each side of the 64 lane vectors are loaded
 // once into LDS (double-buffered over the fields); the lanes then read their x- and z-taps from LDS.
@@ -663,6 +701,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
         const V vrho = m.by_row ? V(m.rho_k[k]) : vload(m.rho + q), vlam = m.by_row ? V(m.lam_k[k]) : vload(m.lam + q),
                 vmu = m.by_row ? V(m.mu_k[k]) : vload(m.mu + q);
         R kv[MECH ? 6 : 5][NV];
+        R ecur[3][NV]; // this stage's strains exx, ezz, exz (memory rebuild only)
         const bool src = m.si1 >= 0 && k >= m.sk0 && k <= m.sk1 && i0 + NV - 1 >= m.si0 && i0 <= m.si1;
 #pragma unroll
         for (int j = 0; j < NV; ++j) {
@@ -713,12 +752,52 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
             kv[3][j] = mad(lambda, div, 2 * mu * dz[1]);
             kv[4][j] = mu * (dz[0] + dx[1]);
             if constexpr (MECH > 0) kv[5][j] = lambda * div;
+            ecur[0][j] = dx[0], ecur[1][j] = dz[1], ecur[2][j] = dz[0] + dx[1];
         }
         // GSLS, in three passes so that few rates are live at once: (1) the stress rates take each mechanism's
         // memories in turn (the reference's order); (2) velocities and stresses are combined, stored and
         // inspected; (3) mechanism by mechanism, the memories' rates are formed and their four fields combined.
         // Pass 3 reads the memories again, from the caches, instead of holding them in registers.
-        if constexpr (MECH > 0) {
+        // strains of the step's earlier stages (memory rebuild only); loaded again where needed after the
+        // stores of this stage, so that they are not held in registers across them
+        const auto load_strains = [&](R (&e3)[3][3][NV]) {
+#pragma unroll
+            for (int s = 0; s < 3; ++s)
+                if (s < A.stage) {
+                    const R *src3 = s < 2 ? A.acc + (size_t)(6 + 3 * s) * cnt : A.in + (size_t)6 * cnt;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const V e = vload(src3 + (size_t)c * cnt + q);
+#pragma unroll
+                        for (int j = 0; j < NV; ++j) e3[s][c][j] = e[j];
+                    }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+#pragma unroll
+                        for (int j = 0; j < NV; ++j) e3[s][c][j] = 0;
+                }
+        };
+        if constexpr (kRebuildMemory<R, MECH>) {
+            R eprev[3][3][NV];
+            load_strains(eprev);
+#pragma unroll 1
+            for (int l = 0; l < MECH; ++l) {
+                const V vdp = m.by_row ? V(m.dP_k[l][k]) : vload(m.dP[l] + q), vds = m.by_row ? V(m.dS_k[l][k]) : vload(m.dS[l] + q);
+#pragma unroll
+                for (int f = 0; f < 3; ++f) { // xx, zz, xz (the yy memories enter Syy only)
+                    const V m0 = vload(A.base + (size_t)(6 + 4 * l + f) * cnt + q);
+#pragma unroll
+                    for (int j = 0; j < NV; ++j) {
+                        R d[3], ks, ms;
+#pragma unroll
+                        for (int s = 0; s < 3; ++s)
+                            d[s] = memory_drive(f, vdp[j], vds[j], eprev[s][0][j], eprev[s][1][j], eprev[s][2][j]);
+                        kv[2 + f][j] -= memory_stage(A.stage, m.rates[l], m0[j], d, A.hdt, A.dt, ks, ms);
+                    }
+                }
+            }
+        } else if constexpr (MECH > 0) {
 #pragma unroll
             for (int l = 0; l < MECH; ++l)
 #pragma unroll
@@ -731,7 +810,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
         {
             R nxt[5][NV];
 #pragma unroll
-            for (int f = 0; f < (MECH ? 6 : 5); ++f) {
+            for (int f = 0; f < (MECH && !kRebuildMemory<R, MECH> ? 6 : 5); ++f) {
                 const size_t o = (size_t)f * cnt + q;
                 // stage input: the stencil centre; Syy's only where FP32 stage 2 rebuilds Y1 from it
                 const V vin = f < 5 ? centre[f < 5 ? f : 0] : (kFast<R> && A.ystage ? vload(A.in + o) : V{});
@@ -754,7 +833,60 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
                 }
             }
         }
-        if constexpr (MECH > 0) {
+        if constexpr (kRebuildMemory<R, MECH>) {
+            if (A.stage < 3) { // this stage's strains, for the later stages of the step
+                R *dst = A.stage < 2 ? A.acc + (size_t)(6 + 3 * A.stage) * cnt : A.out + (size_t)6 * cnt;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    V e;
+#pragma unroll
+                    for (int j = 0; j < NV; ++j) e[j] = ecur[c][j];
+                    vstore(dst + (size_t)c * cnt + q, e, A.nt);
+                }
+            } else { // U^{n+1} of every memory variable and of Syy
+                R eprev[3][3][NV], yy[NV];
+                load_strains(eprev);
+#pragma unroll
+                for (int j = 0; j < NV; ++j) yy[j] = 0;
+#pragma unroll 1
+                for (int l = 0; l < MECH; ++l) {
+                    const V vdp = m.by_row ? V(m.dP_k[l][k]) : vload(m.dP[l] + q), vds = m.by_row ? V(m.dS_k[l][k]) : vload(m.dS[l] + q);
+#pragma unroll
+                    for (int f = 0; f < 4; ++f) {
+                        const size_t o = (size_t)(6 + 4 * l + f) * cnt + q;
+                        const V m0 = vload(A.base + o);
+                        V mn;
+#pragma unroll
+                        for (int j = 0; j < NV; ++j) {
+                            R d[3], ks, ms;
+#pragma unroll
+                            for (int s = 0; s < 3; ++s)
+                                d[s] = memory_drive(f, vdp[j], vds[j], eprev[s][0][j], eprev[s][1][j], eprev[s][2][j]);
+                            const R m3 = memory_stage(3, m.rates[l], m0[j], d, A.hdt, A.dt, ks, ms);
+                            const R k4 = m.rates[l] * (memory_drive(f, vdp[j], vds[j], ecur[0][j], ecur[1][j], ecur[2][j]) - m3);
+                            mn[j] = mad(A.dt6, ks + k4, m0[j]);
+                            if (!isfinite(mn[j])) p.bad = 1;
+                            if (f == 3) yy[j] += ms + m3; // m_0 + 2 m_1 + 2 m_2 + m_3 of the yy memory
+                        }
+                        vstore(A.out + o, mn, A.nt);
+                    }
+                }
+                // Syy: k_s = lambda div_s - (sum of the yy memories' stage inputs), so that
+                // U^{n+1} = U^n + dt / 6 (lambda (div_1 + 2 div_2 + 2 div_3 + div_4) - sum of their weighted sums)
+                const size_t o = (size_t)5 * cnt + q;
+                const V s0 = vload(A.base + o);
+                V sn;
+#pragma unroll
+                for (int j = 0; j < NV; ++j) {
+                    R dsum = ecur[0][j] + ecur[1][j];
+#pragma unroll
+                    for (int s = 0; s < 3; ++s) dsum = mad(s ? (R)2 : (R)1, eprev[s][0][j] + eprev[s][1][j], dsum);
+                    sn[j] = mad(A.dt6, mad(vlam[j], dsum, -yy[j]), s0[j]);
+                    if (!isfinite(sn[j])) p.bad = 1;
+                }
+                vstore(A.out + o, sn, A.nt);
+            }
+        } else if constexpr (MECH > 0) {
 #pragma unroll
             for (int l = 0; l < MECH; ++l) {
                 const int kk = 6 + 4 * l;

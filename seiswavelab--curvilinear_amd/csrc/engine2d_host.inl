@@ -26,6 +26,7 @@ template <class R> struct Engine2D {
     int nBorder = 0, nGeneral = 0; // nodes outside the interior rectangle, blocks of 256 of them
     int rtx = 0, nInterior = 0;   // interior tiles (64*NV x 4 nodes)
     int nPartials = 0, nLevel2 = 0;
+    static constexpr int kDirectPartials = 2048;
     hipEvent_t fork[4]{}, join[4]{};
     int cur = 0, prev = 0, diag_rows = 0, registers = 0, nt = 1, metricOk = 0;
     int tileLds = 0; // elastic interior: stencil input staged through LDS tiles
@@ -355,6 +356,7 @@ template <class R> struct Engine2D {
         a.forcing = (R)(scale * pulse(times[s], frequency));
         a.w = weights[s];
         a.wy = (R)(s == 0 || s == 3 ? dt / 6 : dt / 3);
+        a.hdt = (R)(dt / 2), a.dt = (R)dt, a.dt6 = (R)(dt / 6), a.stage = s;
         a.kind = s == 0 ? 0 : s == 3 ? 2 : 1;
         a.ystage = s == 1;
         a.status = status.as<int>();
@@ -510,10 +512,14 @@ template <class R> struct Engine2D {
             const int n = enqueue_step(c, dt, frequency, scale, st);
             const bool timed = j % 32 == 0; // finalisation is timed on every 32nd step only (events add gaps)
             if (timed) mark(1, true);
-            reduce_kernel<<<nLevel2, THREADS, 0, stream>>>(partials.as<Partial>(), nPartials, level2.as<Partial>(),
-                                                          status.as<int>());
+            // up to kDirectPartials tile partials the finalisation block reduces them itself (one launch
+            // fewer per step); beyond, a first level spreads the reduction over up to 128 blocks
+            const bool direct = nPartials <= kDirectPartials;
+            if (!direct)
+                reduce_kernel<<<nLevel2, THREADS, 0, stream>>>(partials.as<Partial>(), nPartials, level2.as<Partial>(),
+                                                              status.as<int>());
             FinalArgs<R> f{};
-            f.partials = level2.as<Partial>(), f.n = nLevel2;
+            f.partials = direct ? partials.as<Partial>() : level2.as<Partial>(), f.n = direct ? nPartials : nLevel2;
             f.status = status.as<int>(), f.accepted = accepted.as<int>(), f.peak = peak.as<double>();
             f.diag = diag.as<double>(), f.samples = samples.as<double>();
             f.state = S(n), f.probes = probes.as<long long>(), f.nrec = nrec, f.row = j, f.step = st;
