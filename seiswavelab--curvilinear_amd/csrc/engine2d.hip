@@ -210,18 +210,61 @@ __device__ inline R stretched(const DMesh<R> &m, const R *__restrict__ pin, long
     return d / kappa + psi;
 }
 
+// FP32 with two or more GSLS mechanisms: a memory variable obeys the pointwise ODE dm/dt = r (d - m)
+// (- damp m in a sponge), whose drive d is a combination of three strains (exx, ezz, exz) with the
+// mechanism's dP, dS.  Its stage inputs are therefore rebuilt from U^n and the strains of the step's earlier
+// stages instead of being streamed: each of stages 1-3 stores the node's three strains (stages 1 and 2 in
+// the accumulator's memory slots, stage 3 in its output's, which is stage 4's input), and stage 4 forms
+// k1..k4 and writes U^{n+1} = U^n + dt (k1 + 2 k2 + 2 k3 + k4) / 6.  Syy is pointwise too and no other
+// rate reads it (dSyy/dt = lambda div - sum of the yy memories - damp Syy), so it and the yy memories are
+// formed at stage 4 only.  The same RK4, rounded differently; per interior node and step the 13 x 14
+// accesses of the memories and Syy become 80 (three mechanisms).  Every node of the grid (interior and
+// border) follows this scheme, so the scratch slots are per node.
+template <class R, int MECH> constexpr bool kRebuildMemory = kFast<R> && MECH >= 2;
+// drive of memory component f (xx, zz, xz, yy) for a mechanism with moduli dP, dS
+template <class R> __device__ __forceinline__ R memory_drive(int f, R dp, R ds, R exx, R ezz, R exz) {
+    const R dl = dp - 2 * ds, div = exx + ezz;
+    return f == 0 ? mad(dl, div, 2 * ds * exx) : f == 1 ? mad(dl, div, 2 * ds * ezz) : f == 2 ? ds * exz : dl * div;
+}
+// rate of a memory variable (DAMP: with the sponge term)
+template <class R, bool DAMP> __device__ __forceinline__ R memory_rate(R r, R dmp, R d, R m) {
+    if constexpr (DAMP) return mad(-dmp, m, r * (d - m));
+    else return r * (d - m);
+}
+// the stage input m_s of a memory variable after s earlier stages with drives d[0..s-1]; with s = 3 also
+// the weighted sums k1 + 2 k2 + 2 k3 of its rates and m_0 + 2 m_1 + 2 m_2 of its stage inputs
+template <class R, bool DAMP = false>
+__device__ __forceinline__ R memory_stage(int s, R r, R dmp, R m0, const R (&d)[3], R hdt, R dt, R &ksum, R &msum) {
+    R m = m0;
+    ksum = 0, msum = m0;
+    if (s >= 1) {
+        const R k = memory_rate<R, DAMP>(r, dmp, d[0], m);
+        ksum = k, m = mad(hdt, k, m0), msum = mad((R)2, m, msum);
+    }
+    if (s >= 2) {
+        const R k = memory_rate<R, DAMP>(r, dmp, d[1], m);
+        ksum = mad((R)2, k, ksum), m = mad(hdt, k, m0), msum = mad((R)2, m, msum);
+    }
+    if (s >= 3) {
+        const R k = memory_rate<R, DAMP>(r, dmp, d[2], m);
+        ksum = mad((R)2, k, ksum), m = mad(dt, k, m0);
+    }
+    return m;
+}
+
 // traction-free closure for the surface z-derivatives, evaluated in FP64 (PointKernels2D.hpp:79-89)
+// memIn: the memory variables' stage inputs (xx, zz, xz, yy per mechanism) where they are rebuilt, else null
 template <class R>
 __device__ inline void closure(const DMesh<R> &m, const R *__restrict__ w, size_t count, ptrdiff_t q, int i, R xi0, R xi1,
-                               double *result) {
+                               double *result, const R (*memIn)[4] = nullptr) {
     const double c0 = m.sa[i], c1 = m.sb[i], c2 = c0 * c0 + c1 * c1, length = sqrt(c2);
     const double n0 = c0 / length, n1 = c1 / length;
     const double dx = xi0, dz = xi1, lambda = m.slam[i], mu = m.smu[i];
     double t0 = (lambda + 2 * mu) * dx * c0 + mu * dz * c1, t1 = mu * dz * c0 + lambda * dx * c1;
     for (int l = 0; l < m.mech; ++l) {
         const int k = 6 + 4 * l;
-        const double sxx = w[(size_t)k * count + q], szz = w[(size_t)(k + 1) * count + q],
-                     sxz = w[(size_t)(k + 2) * count + q];
+        const double sxx = memIn ? memIn[l][0] : w[(size_t)k * count + q], szz = memIn ? memIn[l][1] : w[(size_t)(k + 1) * count + q],
+                     sxz = memIn ? memIn[l][2] : w[(size_t)(k + 2) * count + q];
         t0 -= sxx * c0 + sxz * c1;
         t1 -= sxz * c0 + szz * c1;
     }
@@ -230,10 +273,13 @@ __device__ inline void closure(const DMesh<R> &m, const R *__restrict__ w, size_
     result[1] = -(t1 - n1 * normal) / (mu * c2) - n1 * normal / ((lambda + 2 * mu) * c2);
 }
 
-// Full RHS of one node outside the interior rectangle (PointKernels2D.hpp:121-172).
-template <class R, int DX, int DZ, int MECH>
+// Full RHS of one node outside the interior rectangle (PointKernels2D.hpp:121-172).  REBUILD: the memory
+// variables' stage inputs come from memIn (the rates of the memories, Syy and their sponge terms are then
+// left to the caller) and the node's strains exx, ezz, exz go to strain.
+template <class R, int DX, int DZ, int MECH, bool REBUILD = false>
 __device__ inline void rhs_general(const DMesh<R> &m, const R *__restrict__ w, const R *__restrict__ pin, size_t count,
-                                   int i, int k, R forcing, R *kout, R *prate, long long &sx, long long &sz) {
+                                   int i, int k, R forcing, R *kout, R *prate, long long &sx, long long &sz,
+                                   const R (*memIn)[4] = nullptr, R *strain = nullptr) {
     constexpr int F = MECH ? 6 + 4 * MECH : 5;
     const int depth = k - 3;
     const ptrdiff_t q = at(m, i, k), s = m.ld;
@@ -250,7 +296,7 @@ __device__ inline void rhs_general(const DMesh<R> &m, const R *__restrict__ w, c
         R xs[2];
         for (int v = 0; v < 2; ++v) xs[v] = stretched(m, pin, sx, i - 3, 0, v, xi[v], prate);
         double sur[2];
-        closure(m, w, count, q, i, xs[0], xs[1], sur);
+        closure(m, w, count, q, i, xs[0], xs[1], sur, REBUILD ? memIn : nullptr);
         eta[0] = (R)sur[0];
         eta[1] = (R)sur[1];
     } else {
@@ -323,7 +369,7 @@ __device__ inline void rhs_general(const DMesh<R> &m, const R *__restrict__ w, c
             const R drive[4] = {dl * div + 2 * ds * dx[0], dl * div + 2 * ds * dz[1], ds * (dz[0] + dx[1]), dl * div};
             R mem[4];
 #pragma unroll
-            for (int f = 0; f < 4; ++f) mem[f] = w[(size_t)(kk + f) * count + q];
+            for (int f = 0; f < 4; ++f) mem[f] = REBUILD ? memIn[l][f] : w[(size_t)(kk + f) * count + q];
             kout[2] -= mem[0];
             kout[3] -= mem[1];
             kout[4] -= mem[2];
@@ -335,8 +381,9 @@ __device__ inline void rhs_general(const DMesh<R> &m, const R *__restrict__ w, c
     if (m.has_damp) {
         const R d = m.damp[q];
 #pragma unroll
-        for (int f = 0; f < F; ++f) kout[f] -= d * w[(size_t)f * count + q];
+        for (int f = 0; f < (REBUILD ? 5 : F); ++f) kout[f] -= d * w[(size_t)f * count + q];
     }
+    if constexpr (REBUILD) strain[0] = dx[0], strain[1] = dz[1], strain[2] = dz[0] + dx[1];
 }
 
 // surface stress projection sigma <- P sigma P on row k = 3 (PointKernels2D.hpp:70-78), FP64
@@ -418,12 +465,40 @@ __device__ __forceinline__ void general_body(const StageArgs<R> &A, int li, Part
     if (live) {
         R kv[F], pr[8];
         long long sx, sz;
-        rhs_general<R, DX, DZ, MECH>(m, A.in, A.pin, A.count, i, k, A.forcing, kv, pr, sx, sz);
         const ptrdiff_t q = at(m, i, k);
         R nxt[F];
+        constexpr bool REB = kRebuildMemory<R, MECH>;
+        constexpr int FW = REB ? 5 : F; // fields combined here (the wavefields, where the rest is rebuilt)
+        R ep[3][3], strain[3];          // strains of the step's earlier stages and of this one (rebuild)
+        const size_t cnt = A.count;
+        const R dmp = REB && m.has_damp ? m.damp[q] : (R)0;
+        if constexpr (REB) {
 #pragma unroll
-        for (int f = 0; f < F; ++f) {
-            const size_t o = (size_t)f * A.count + q;
+            for (int s = 0; s < 3; ++s)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    ep[s][c] = s < A.stage ? (s < 2 ? A.acc[(size_t)(6 + 3 * s + c) * cnt + q] : A.in[(size_t)(6 + c) * cnt + q]) : (R)0;
+            R memIn[MECH][4];
+#pragma unroll
+            for (int l = 0; l < MECH; ++l) {
+                const R dp = m.dP[l][q], ds = m.dS[l][q];
+#pragma unroll
+                for (int f = 0; f < 3; ++f) {
+                    R d[3], ks, ms;
+#pragma unroll
+                    for (int s = 0; s < 3; ++s) d[s] = memory_drive(f, dp, ds, ep[s][0], ep[s][1], ep[s][2]);
+                    memIn[l][f] = memory_stage<R, true>(A.stage, m.rates[l], dmp, A.base[(size_t)(6 + 4 * l + f) * cnt + q], d,
+                                                        A.hdt, A.dt, ks, ms);
+                }
+                memIn[l][3] = 0; // the yy memories enter Syy only, which is formed at the last stage
+            }
+            rhs_general<R, DX, DZ, MECH, true>(m, A.in, A.pin, cnt, i, k, A.forcing, kv, pr, sx, sz, memIn, strain);
+        } else {
+            rhs_general<R, DX, DZ, MECH>(m, A.in, A.pin, cnt, i, k, A.forcing, kv, pr, sx, sz);
+        }
+#pragma unroll
+        for (int f = 0; f < FW; ++f) {
+            const size_t o = (size_t)f * cnt + q;
             R acc = A.kind == 0 ? (R)0 : A.acc[o];
             nxt[f] = combine(A.kind, A.base[o], kv[f], A.w, acc);
             if (A.kind != 2) A.acc[o] = acc;
@@ -434,7 +509,48 @@ __device__ __forceinline__ void general_body(const StageArgs<R> &A, int li, Part
             nxt[2] = (R)sxx, nxt[3] = (R)szz, nxt[4] = (R)sxz;
         }
 #pragma unroll
-        for (int f = 0; f < F; ++f) A.out[(size_t)f * A.count + q] = nxt[f];
+        for (int f = 0; f < FW; ++f) A.out[(size_t)f * cnt + q] = nxt[f];
+        if constexpr (REB) {
+            if (A.stage < 3) { // this stage's strains, for the later stages of the step
+                R *dst = A.stage < 2 ? A.acc + (size_t)(6 + 3 * A.stage) * cnt : A.out + (size_t)6 * cnt;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) dst[(size_t)c * cnt + q] = strain[c];
+            } else { // U^{n+1} of the memories and of Syy, with the sponge terms
+                R yy[4] = {0, 0, 0, 0}; // sum over mechanisms of the yy memories' stage inputs
+#pragma unroll
+                for (int l = 0; l < MECH; ++l) {
+                    const R dp = m.dP[l][q], ds = m.dS[l][q], r = m.rates[l];
+#pragma unroll
+                    for (int f = 0; f < 4; ++f) {
+                        const size_t o = (size_t)(6 + 4 * l + f) * cnt + q;
+                        R d[4];
+#pragma unroll
+                        for (int s = 0; s < 3; ++s) d[s] = memory_drive(f, dp, ds, ep[s][0], ep[s][1], ep[s][2]);
+                        d[3] = memory_drive(f, dp, ds, strain[0], strain[1], strain[2]);
+                        const R m0 = A.base[o];
+                        const R k1 = memory_rate<R, true>(r, dmp, d[0], m0), m1 = mad(A.hdt, k1, m0);
+                        const R k2 = memory_rate<R, true>(r, dmp, d[1], m1), m2 = mad(A.hdt, k2, m0);
+                        const R k3 = memory_rate<R, true>(r, dmp, d[2], m2), m3 = mad(A.dt, k3, m0);
+                        const R k4 = memory_rate<R, true>(r, dmp, d[3], m3);
+                        nxt[6 + 4 * l + f] = mad(A.dt6, k1 + 2 * (k2 + k3) + k4, m0);
+                        A.out[o] = nxt[6 + 4 * l + f];
+                        if (f == 3) yy[0] += m0, yy[1] += m1, yy[2] += m2, yy[3] += m3;
+                    }
+                }
+                const size_t o = (size_t)5 * cnt + q;
+                const R lambda = m.lam[q], s0 = A.base[o];
+                R div[4];
+#pragma unroll
+                for (int s = 0; s < 3; ++s) div[s] = ep[s][0] + ep[s][1];
+                div[3] = strain[0] + strain[1];
+                const R k1 = mad(lambda, div[0], -yy[0]) - dmp * s0, s1 = mad(A.hdt, k1, s0);
+                const R k2 = mad(lambda, div[1], -yy[1]) - dmp * s1, s2 = mad(A.hdt, k2, s0);
+                const R k3 = mad(lambda, div[2], -yy[2]) - dmp * s2, s3 = mad(A.dt, k3, s0);
+                const R k4 = mad(lambda, div[3], -yy[3]) - dmp * s3;
+                nxt[5] = mad(A.dt6, k1 + 2 * (k2 + k3) + k4, s0);
+                A.out[o] = nxt[5];
+            }
+        }
         R pn[8];
         const long long slots[2] = {sx, sz};
 #pragma unroll
@@ -533,42 +649,6 @@ __device__ __forceinline__ void combine_field(const StageArgs<R> &A, size_t o, b
     }
     vstore(A.out + o, vo, A.nt);
     if (A.kind == 1 || (A.kind == 0 && !kFast<R>)) vstore(A.acc + o, vacc, A.nt);
-}
-// FP32 interior nodes with two or more GSLS mechanisms: a memory variable obeys the pointwise ODE
-// dm/dt = r (d - m), whose drive d is a combination of three strains (exx, ezz, exz) with the mechanism's
-// dP, dS.  Its stage inputs are therefore rebuilt from U^n and the strains of the step's earlier stages
-// instead of being streamed: each of stages 1-3 stores its three strains (stages 1 and 2 in the
-// accumulator's memory slots, stage 3 in its output's, which is stage 4's input), and stage 4 forms
-// k1..k4 and writes U^{n+1} = U^n + dt (k1 + 2 k2 + 2 k3 + k4) / 6.  Syy is pointwise too and no other
-// rate reads it (dSyy/dt = lambda div - sum of the yy memories), so it and the yy memories are formed at
-// stage 4 only.  The same RK4, rounded differently; per node and step the 13 x 14 accesses of the
-// memories and Syy become 80 (three mechanisms).  Border nodes stream theirs as before; the two sets of
-// nodes are disjoint and the scratch slots are per node.
-template <class R, int MECH> constexpr bool kRebuildMemory = kFast<R> && MECH >= 2;
-// drive of memory component f (xx, zz, xz, yy) for a mechanism with moduli dP, dS
-template <class R> __device__ __forceinline__ R memory_drive(int f, R dp, R ds, R exx, R ezz, R exz) {
-    const R dl = dp - 2 * ds, div = exx + ezz;
-    return f == 0 ? mad(dl, div, 2 * ds * exx) : f == 1 ? mad(dl, div, 2 * ds * ezz) : f == 2 ? ds * exz : dl * div;
-}
-// the stage input m_s of a memory variable after s earlier stages with drives d[0..s-1]; with s = 3 also
-// the weighted sums k1 + 2 k2 + 2 k3 of its rates and m_0 + 2 m_1 + 2 m_2 of its stage inputs
-template <class R>
-__device__ __forceinline__ R memory_stage(int s, R r, R m0, const R (&d)[3], R hdt, R dt, R &ksum, R &msum) {
-    R m = m0;
-    ksum = 0, msum = m0;
-    if (s >= 1) {
-        const R k = r * (d[0] - m);
-        ksum = k, m = mad(hdt, k, m0), msum = mad((R)2, m, msum);
-    }
-    if (s >= 2) {
-        const R k = r * (d[1] - m);
-        ksum = mad((R)2, k, ksum), m = mad(hdt, k, m0), msum = mad((R)2, m, msum);
-    }
-    if (s >= 3) {
-        const R k = r * (d[2] - m);
-        ksum = mad((R)2, k, ksum), m = mad(dt, k, m0);
-    }
-    return m;
 }
 // LDS staging of the stencil input (the `tile` variant): per field, the block's 4 rows plus the 3 + 1
 // halo rows of the z-stencil (8 rows) and HV halo vectors on each side of the 64 lane vectors are loaded
@@ -793,7 +873,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
 #pragma unroll
                         for (int s = 0; s < 3; ++s)
                             d[s] = memory_drive(f, vdp[j], vds[j], eprev[s][0][j], eprev[s][1][j], eprev[s][2][j]);
-                        kv[2 + f][j] -= memory_stage(A.stage, m.rates[l], m0[j], d, A.hdt, A.dt, ks, ms);
+                        kv[2 + f][j] -= memory_stage(A.stage, m.rates[l], (R)0, m0[j], d, A.hdt, A.dt, ks, ms);
                     }
                 }
             }
@@ -862,7 +942,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
 #pragma unroll
                             for (int s = 0; s < 3; ++s)
                                 d[s] = memory_drive(f, vdp[j], vds[j], eprev[s][0][j], eprev[s][1][j], eprev[s][2][j]);
-                            const R m3 = memory_stage(3, m.rates[l], m0[j], d, A.hdt, A.dt, ks, ms);
+                            const R m3 = memory_stage(3, m.rates[l], (R)0, m0[j], d, A.hdt, A.dt, ks, ms);
                             const R k4 = m.rates[l] * (memory_drive(f, vdp[j], vds[j], ecur[0][j], ecur[1][j], ecur[2][j]) - m3);
                             mn[j] = mad(A.dt6, ks + k4, m0[j]);
                             if (!isfinite(mn[j])) p.bad = 1;

@@ -38,13 +38,17 @@ def gather(rec):
     return np.concatenate([rec["vx"].ravel(), rec["vz"].ravel()])
 
 
-CASES = sorted(REF_CASES) + ["elastic_curved_moment", "gsls3_curved_pml"]
+CASES = sorted(REF_CASES) + ["elastic_curved_moment", "gsls3_curved_pml", "gsls3_curved_sponge"]
 
 
 def small_case(name):
     if name in REF_CASES:
         return ref_case(name)
-    p = precision_case(name)
+    if name == "gsls3_curved_sponge":  # GSLS-3 under the sponge: damped memory variables and Syy
+        p = precision_case("gsls3_curved_pml")
+        p.absorber = Absorber.Sponge
+    else:
+        p = precision_case(name)
     p.maxSteps = 150
     return p
 
@@ -86,6 +90,9 @@ def test_fp32_within_tolerance_of_fp64_oracle(name):
     assert a["vx"].shape == b["vx"].shape
     assert rel_l2(gather(b), gather(a)) <= FP32_GATHER_TOL
     assert s.diagnostics()["tractionRelative"] < 5e-6
+    # every field, Syy and the memory variables included (FP32 GSLS forms them at the last stage only)
+    for f in range(s.layout()["fields"]):
+        assert rel_l2(o.field(f), s.snapshot_field(f)) <= FP32_GATHER_TOL, f
 
 
 def test_reset_is_bitwise_reproducible_and_chunking_invariant():
