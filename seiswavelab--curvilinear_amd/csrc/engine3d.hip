@@ -274,6 +274,8 @@ __device__ __forceinline__ void deep_node(const Stage3<R> &A, long long q, int i
             kv[a] += A.forcing * (m.grad[0] ? m.grad[a][q] : m.src[q] * m.dir[a]) * (m.rho_global / density);
     }
     const R damp = m.damp ? m.damp[q] : (m.dax[i] + m.day[j]) + m.daz[k];
+    // the stage output and the accumulator are not read again within the stage: nontemporal stores keep them
+    // from displacing the y- and z-halo lines of the input in L2 and the Infinity Cache
     R nxt[9];
 #pragma unroll
     for (int f = 0; f < 9; ++f) {
@@ -284,9 +286,9 @@ __device__ __forceinline__ void deep_node(const Stage3<R> &A, long long q, int i
         } else {
             const R b = A.base[o];
             nxt[f] = b + A.w * kf;
-            A.acc[o] = (A.kind == 0 ? b : A.acc[o]) + A.wy * kf;
+            __builtin_nontemporal_store((A.kind == 0 ? b : A.acc[o]) + A.wy * kf, A.acc + o);
         }
-        A.out[o] = nxt[f];
+        __builtin_nontemporal_store(nxt[f], A.out + o);
     }
     if (A.kind == 2) {
         bool bad = false;
