@@ -79,6 +79,7 @@ template <class R> struct Stage3 {
     size_t count;
     R forcing, w, wy; // stage weight and accumulator weight (dt/6, dt/3, dt/3, dt/6)
     int kind;         // 0 first, 1 middle, 2 final (+ inspection)
+    int ystage;       // stage 2 (FP32 deep bands rebuild Y1 = U^n + dt/6 k1 instead of reading it)
     const int *status;
     Partial3 *partials;
     int ntx, nty, ntiles; // tiles of 64x4 nodes per plane row / column, tiles of all planes
@@ -282,11 +283,21 @@ __device__ __forceinline__ void deep_node(const Stage3<R> &A, long long q, int i
         const size_t o = (size_t)f * cnt + q;
         const R kf = kv[f] - damp * w[o];
         if (A.kind == 2) {
-            nxt[f] = A.acc[o] + A.wy * kf;
+            nxt[f] = __builtin_nontemporal_load(A.acc + o) + A.wy * kf;
         } else {
-            const R b = A.base[o];
+            const R b = __builtin_nontemporal_load(A.base + o);
             nxt[f] = b + A.w * kf;
-            __builtin_nontemporal_store((A.kind == 0 ? b : A.acc[o]) + A.wy * kf, A.acc + o);
+            if constexpr (sizeof(R) == 4) {
+                // FP32: stage 1 does not store Y1 = U^n + dt/6 k1; stage 2 rebuilds it from its input
+                // U2 = U^n + dt/2 k1 (unprojected in these bands) as U^n + (U2 - U^n) / 3
+                if (A.kind == 1) {
+                    const R y = A.ystage ? b + (w[o] - b) * (R)(1.0 / 3.0) : __builtin_nontemporal_load(A.acc + o);
+                    __builtin_nontemporal_store(y + A.wy * kf, A.acc + o);
+                }
+            } else {
+                __builtin_nontemporal_store((A.kind == 0 ? b : __builtin_nontemporal_load(A.acc + o)) + A.wy * kf,
+                                            A.acc + o);
+            }
         }
         __builtin_nontemporal_store(nxt[f], A.out + o);
     }

@@ -236,6 +236,7 @@ template <class R> struct Engine3D {
             a.forcing = (R)(scale * (1.0 - 2.0 * tau * tau) * std::exp(-tau * tau));
             a.w = (R)w[s], a.wy = (R)wy[s];
             a.kind = s == 0 ? 0 : s == 3 ? 2 : 1;
+            a.ystage = s == 1;
             a.status = status.as<int>();
             a.partials = partials[part].as<Partial3>();
             a.ntx = ntx, a.nty = nty, a.ntiles = ntiles;
