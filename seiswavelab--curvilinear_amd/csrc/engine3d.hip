@@ -27,7 +27,11 @@
 namespace swl {
 namespace {
 
-constexpr int TX = 64, TY = 4, THREADS = TX * TY;
+// A tile (one block, one k plane) is one 128-byte line of a row in x (16 FP64 / 32 FP32 nodes) by THREADS / that
+// rows in y: a row segment is still one whole cache line, and the taller tile re-reads fewer y-halo rows.
+constexpr int THREADS = 256;
+template <class R> constexpr int TX = 128 / (int)sizeof(R);
+template <class R> constexpr int TY = THREADS / TX<R>;
 constexpr int ORIGIN = 61; // off: column 3 lands on a 64-element boundary
 
 template <class R> struct DMesh3 {
@@ -82,7 +86,7 @@ template <class R> struct Stage3 {
     int ystage;       // stage 2 (FP32 deep bands rebuild Y1 = U^n + dt/6 k1 instead of reading it)
     const int *status;
     Partial3 *partials;
-    int ntx, nty, ntiles; // tiles of 64x4 nodes per plane row / column, tiles of all planes
+    int ntx, nty, ntiles; // tiles of TX x TY nodes per plane row / column, tiles of all planes
     int t0, nband;        // this launch covers tiles [t0, t0 + nband) (a band of k planes)
 };
 
@@ -208,7 +212,7 @@ template <class R> __device__ inline R mat(const DMesh3<R> &m, const R *a, long 
     return m.by_depth ? a[k] : a[q];
 }
 
-// One RK stage over every physical node (one node per lane, 64 x 4 nodes per block, one k plane).
+// One RK stage over every physical node (one node per lane, TX x TY nodes per block, one k plane).
 // Depth bands of one stage: SURFACE = planes k = 3..6 (closure, reduced stencils, traction image; the
 // general per-node path below), TERRAIN = deeper planes with stored metrics, FLAT = planes with identity
 // metrics and J = 1 (below the terrain transition).  The deep bands evaluate the same expressions of
@@ -336,7 +340,7 @@ __global__ __launch_bounds__(THREADS) void stage3_kernel(Stage3<R> A) {
     const int t = A.t0 + (u % planes) * tilesPerPlane + u / planes;
     if constexpr (BAND != SURFACE) {
         const int per = A.ntx * A.nty, kk = t / per, rem = t % per;
-        const int i = 3 + (rem % A.ntx) * TX + (threadIdx.x % TX), j = 3 + (rem / A.ntx) * TY + threadIdx.x / TX, k = 3 + kk;
+        const int i = 3 + (rem % A.ntx) * TX<R> + (threadIdx.x % TX<R>), j = 3 + (rem / A.ntx) * TY<R> + threadIdx.x / TX<R>, k = 3 + kk;
         Partial3 p{};
         if (i < m.nx - 3 && j < m.ny - 3) deep_node<R, DX, DY, DZ, BAND>(A, at3(m, i, j, k), i, j, k, p);
         if (A.kind == 2) {
@@ -346,7 +350,7 @@ __global__ __launch_bounds__(THREADS) void stage3_kernel(Stage3<R> A) {
         return;
     }
     const int per = A.ntx * A.nty, kk = t / per, rem = t % per;
-    const int i = 3 + (rem % A.ntx) * TX + (threadIdx.x % TX), j = 3 + (rem / A.ntx) * TY + threadIdx.x / TX, k = 3 + kk;
+    const int i = 3 + (rem % A.ntx) * TX<R> + (threadIdx.x % TX<R>), j = 3 + (rem / A.ntx) * TY<R> + threadIdx.x / TX<R>, k = 3 + kk;
     Partial3 p{};
     if (i < m.nx - 3 && j < m.ny - 3) {
         const size_t cnt = A.count;

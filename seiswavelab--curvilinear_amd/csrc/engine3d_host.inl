@@ -164,7 +164,7 @@ template <class R> struct Engine3D {
         }
         for (auto &b : state) b.allocate((size_t)9 * count * sizeof(R));
         accbuf.allocate((size_t)9 * count * sizeof(R));
-        ntx = (nx - 6 + TX - 1) / TX, nty = (ny - 6 + TY - 1) / TY;
+        ntx = (nx - 6 + TX<R> - 1) / TX<R>, nty = (ny - 6 + TY<R> - 1) / TY<R>;
         ntiles = ntx * nty * (nz - 6);
         for (auto &b : partials) b.allocate((size_t)ntiles * sizeof(Partial3));
         nLevel2 = std::min(128, std::max(1, ntiles / 64));
