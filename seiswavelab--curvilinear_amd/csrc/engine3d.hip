@@ -230,7 +230,7 @@ __device__ __forceinline__ void deep_node(const Stage3<R> &A, long long q, int i
     R mt[9];
     if constexpr (BAND == TERRAIN) {
 #pragma unroll
-        for (int a = 0; a < 9; ++a) mt[a] = m.met[(size_t)a * m.mcount + q];
+        for (int a = 0; a < 9; ++a) mt[a] = __builtin_nontemporal_load(m.met + (size_t)a * m.mcount + q); // read once
     }
     // derivatives of field f along xi, eta, zeta (DRP) and the requested Cartesian components
     auto deriv = [&](int f, R d[3]) {
