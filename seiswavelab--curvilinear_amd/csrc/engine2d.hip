@@ -210,17 +210,17 @@ __device__ inline R stretched(const DMesh<R> &m, const R *__restrict__ pin, long
     return d / kappa + psi;
 }
 
-// FP32 with two or more GSLS mechanisms: a memory variable obeys the pointwise ODE dm/dt = r (d - m)
+// FP32 with GSLS mechanisms: a memory variable obeys the pointwise ODE dm/dt = r (d - m)
 // (- damp m in a sponge), whose drive d is a combination of three strains (exx, ezz, exz) with the
 // mechanism's dP, dS.  Its stage inputs are therefore rebuilt from U^n and the strains of the step's earlier
-// stages instead of being streamed: each of stages 1-3 stores the node's three strains (stages 1 and 2 in
-// the accumulator's memory slots, stage 3 in its output's, which is stage 4's input), and stage 4 forms
+// stages instead of being streamed: each of stages 1-3 stores the node's three strains in otherwise unused
+// memory slots (strain_dst / strain_src below), and stage 4 forms
 // k1..k4 and writes U^{n+1} = U^n + dt (k1 + 2 k2 + 2 k3 + k4) / 6.  Syy is pointwise too and no other
 // rate reads it (dSyy/dt = lambda div - sum of the yy memories - damp Syy), so it and the yy memories are
 // formed at stage 4 only.  The same RK4, rounded differently; per interior node and step the 13 x 14
 // accesses of the memories and Syy become 80 (three mechanisms).  Every node of the grid (interior and
 // border) follows this scheme, so the scratch slots are per node.
-template <class R, int MECH> constexpr bool kRebuildMemory = kFast<R> && MECH >= 2;
+template <class R, int MECH> constexpr bool kRebuildMemory = kFast<R> && MECH >= 1;
 // drive of memory component f (xx, zz, xz, yy) for a mechanism with moduli dP, dS
 template <class R> __device__ __forceinline__ R memory_drive(int f, R dp, R ds, R exx, R ezz, R exz) {
     const R dl = dp - 2 * ds, div = exx + ezz;
@@ -250,6 +250,21 @@ __device__ __forceinline__ R memory_stage(int s, R r, R dmp, R m0, const R (&d)[
         ksum = mad((R)2, k, ksum), m = mad(dt, k, m0);
     }
     return m;
+}
+// Memory slots that hold the strains of stage s (0..2) of the step.  With two or more mechanisms: stages 1 and 2
+// in the accumulator's slots 6 + 3 s, stage 3 in its output's slots 6..8 (stage 4's input).  With one mechanism the
+// accumulator has four memory slots: stage 1 there, stage 2 in its output's (stage 3's input, stage 4's output,
+// read there before stage 4 writes the memories), stage 3 as above.  Stage buffers: in = U^n, b1, b2, b1 and
+// out = b1, b2, b1, b2 for stages 1..4.
+template <class R, int MECH> __device__ __forceinline__ R *strain_dst(const StageArgs<R> &A) {
+    const int s = A.stage;
+    if (s == 0 || (s == 1 && MECH >= 2)) return A.acc + (size_t)(6 + 3 * s) * A.count;
+    return A.out + (size_t)6 * A.count;
+}
+template <class R, int MECH> __device__ __forceinline__ const R *strain_src(const StageArgs<R> &A, int s) {
+    if (s == 0 || (s == 1 && MECH >= 2)) return A.acc + (size_t)(6 + 3 * s) * A.count;
+    if (s == 1) return (A.stage == 2 ? A.in : A.out) + (size_t)6 * A.count;
+    return A.in + (size_t)6 * A.count;
 }
 
 // traction-free closure for the surface z-derivatives, evaluated in FP64 (PointKernels2D.hpp:79-89)
@@ -477,7 +492,7 @@ __device__ __forceinline__ void general_body(const StageArgs<R> &A, int li, Part
             for (int s = 0; s < 3; ++s)
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
-                    ep[s][c] = s < A.stage ? (s < 2 ? A.acc[(size_t)(6 + 3 * s + c) * cnt + q] : A.in[(size_t)(6 + c) * cnt + q]) : (R)0;
+                    ep[s][c] = s < A.stage ? strain_src<R, MECH>(A, s)[(size_t)c * cnt + q] : (R)0;
             R memIn[MECH][4];
 #pragma unroll
             for (int l = 0; l < MECH; ++l) {
@@ -512,7 +527,7 @@ __device__ __forceinline__ void general_body(const StageArgs<R> &A, int li, Part
         for (int f = 0; f < FW; ++f) A.out[(size_t)f * cnt + q] = nxt[f];
         if constexpr (REB) {
             if (A.stage < 3) { // this stage's strains, for the later stages of the step
-                R *dst = A.stage < 2 ? A.acc + (size_t)(6 + 3 * A.stage) * cnt : A.out + (size_t)6 * cnt;
+                R *dst = strain_dst<R, MECH>(A);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) dst[(size_t)c * cnt + q] = strain[c];
             } else { // U^{n+1} of the memories and of Syy, with the sponge terms
@@ -844,7 +859,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
 #pragma unroll
             for (int s = 0; s < 3; ++s)
                 if (s < A.stage) {
-                    const R *src3 = s < 2 ? A.acc + (size_t)(6 + 3 * s) * cnt : A.in + (size_t)6 * cnt;
+                    const R *src3 = strain_src<R, MECH>(A, s);
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
                         const V e = vload(src3 + (size_t)c * cnt + q);
@@ -915,7 +930,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
         }
         if constexpr (kRebuildMemory<R, MECH>) {
             if (A.stage < 3) { // this stage's strains, for the later stages of the step
-                R *dst = A.stage < 2 ? A.acc + (size_t)(6 + 3 * A.stage) * cnt : A.out + (size_t)6 * cnt;
+                R *dst = strain_dst<R, MECH>(A);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     V e;

@@ -38,7 +38,7 @@ def gather(rec):
     return np.concatenate([rec["vx"].ravel(), rec["vz"].ravel()])
 
 
-CASES = sorted(REF_CASES) + ["elastic_curved_moment", "gsls3_curved_pml", "gsls3_curved_sponge"]
+CASES = sorted(REF_CASES) + ["elastic_curved_moment", "gsls1_curved_pml", "gsls3_curved_pml", "gsls3_curved_sponge"]
 
 
 def small_case(name):
