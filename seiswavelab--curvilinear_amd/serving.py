@@ -86,9 +86,12 @@ class Server:
     """budget_bytes: HBM admission budget; chunk: steps between admission / completion checks."""
 
     def __init__(self, device: int = 0, budget_bytes: float = 200e9, chunk: int = 32, builders: int = 6,
-                 keep_traces: bool = True):
+                 keep_traces: bool = True, largest_first: bool = True):
         self.device, self.budget, self.chunk, self.builders = device, budget_bytes, chunk, builders
         self.keep_traces = keep_traces
+        # requests admitted together (a backlog) are built largest first, so that the last sessions to join
+        # the batch are the quick ones; admission itself stays in arrival order
+        self.largest_first = largest_first
 
     @staticmethod
     def _build(p: Parameters, device: int):
@@ -111,6 +114,7 @@ class Server:
         now = lambda: time.perf_counter() - t0  # noqa: E731
         while queue or building or running:
             # admission: arrived requests, FIFO, while the footprint fits the budget
+            admitted = []
             while queue and queue[0][1] <= now():
                 i, arrival, p = queue[0]
                 need = footprint_bytes(p)
@@ -122,6 +126,10 @@ class Server:
                 self.max_used = max(self.max_used, used)
                 r = results[i]
                 r.admitted, r.footprint = now(), need
+                admitted.append((need, i, p))
+            if self.largest_first:
+                admitted.sort(key=lambda x: -x[0])
+            for _, i, p in admitted:
                 building[i] = pool.submit(self._build, p, self.device)
             # built sessions join the batch between chunks
             for i in [i for i, f in building.items() if f.done()]:
