@@ -255,7 +255,7 @@ def measure_serving(pkg, count, steps, rate=20.0):
     from seiswavelab_curvilinear_amd import serving, workloads
     reqs = list(workloads.serving_requests(count=count, rate=rate, steps=steps))
     window = reqs[-1][1]
-    served = serving.Server(chunk=32).run(copy.deepcopy(reqs))
+    served = serving.Server().run(copy.deepcopy(reqs))
     log(f"[bench] serving: {count} requests, makespan {served.wall:.2f} s; {served.summary()}")
     plain = serving.serve_sequential(copy.deepcopy(reqs), precision=pkg.Precision.Float64)
     log(f"[bench] plain FP64 sequential: makespan {plain.wall:.2f} s; {plain.summary()}")
@@ -268,7 +268,7 @@ def measure_serving(pkg, count, steps, rate=20.0):
     s, q = served.summary(), plain.summary()
     # capacity: the same requests all present at t = 0, so neither path waits for arrivals
     burst = [(i, 0.0, p) for i, _, p in copy.deepcopy(reqs)]
-    served_sat = serving.Server(chunk=32).run(copy.deepcopy(burst))
+    served_sat = serving.Server().run(copy.deepcopy(burst))
     plain_sat = serving.serve_sequential(copy.deepcopy(burst), precision=pkg.Precision.Float64, keep_traces=False)
     ss, qs = served_sat.summary(), plain_sat.summary()
     log(f"[bench] serving at saturation: served {ss}; plain FP64 {qs}")

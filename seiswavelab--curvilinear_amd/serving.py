@@ -85,7 +85,7 @@ class ServeReport:
 class Server:
     """budget_bytes: HBM admission budget; chunk: steps between admission / completion checks."""
 
-    def __init__(self, device: int = 0, budget_bytes: float = 200e9, chunk: int = 32, builders: int = 4,
+    def __init__(self, device: int = 0, budget_bytes: float = 200e9, chunk: int = 64, builders: int = 4,
                  keep_traces: bool = True, largest_first: bool = True):
         self.device, self.budget, self.chunk, self.builders = device, budget_bytes, chunk, builders
         self.keep_traces = keep_traces
