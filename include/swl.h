@@ -189,11 +189,15 @@ int swl_engine2d_run(swl_engine2d *e, int first_step, int count, double dt, doub
    back-to-back stage launches of a step; the finalisation is timed on every 32nd step.
    enable resets the counters. */
 int swl_engine2d_profile(swl_engine2d *e, int enable);
-/* out[9]: stage-kernel ms, stage launches, finalisation ms, finalisation launches, interior tiles,
-   stage-kernel VGPRs, computational nodes per stage launch, border tiles, on-the-fly metrics (0/1). */
+/* out[10]: stage-kernel ms, stage launches, finalisation ms, finalisation launches, interior tiles,
+   stage-kernel VGPRs, computational nodes per stage launch, border tiles, on-the-fly metrics (0/1),
+   per-row material tables (1: lambda, mu, rho, dP, dS depend on the row only; 0: per-node arrays). */
 int swl_engine2d_kernel_timing(swl_engine2d *e, double *out);
 /* Engine tuning switches: "nontemporal_stores" (default 1) for the interior outputs;
-   "stored_metrics" (default 0) reads a, b, J from memory instead of recomputing them. */
+   "stored_metrics" (default 0) reads a, b, J from memory instead of recomputing them;
+   "lds_tiles" (default 0) stages the elastic interior's stencil input through LDS;
+   "material_rows" (default 1 where the materials depend on the row only) reads the per-row tables,
+   0 reads the per-node arrays. */
 int swl_engine2d_set_option(swl_engine2d *e, const char *name, int value);
 /* Measurement utility (no reference counterpart): `repeats` plain streaming copies of `bytes`
    (read `bytes`, write `bytes`) on `device`; ms_per_copy = mean HIP-event time per copy.  Used to
@@ -212,6 +216,11 @@ void swl_sim2d_destroy(swl_sim2d *s);
 int swl_sim2d_advance(swl_sim2d *s, int count);
 int swl_sim2d_reset(swl_sim2d *s);
 int swl_sim2d_step(const swl_sim2d *s);
+/* Replaces the accepted peak energy that the late-growth fault rule compares against
+   (!finite || (t > 3/f && peak > 0 && E > 1e4*peak), CurvilinearSolver2D.cpp:576-578, where the
+   reference keeps it private).  Lets a caller restore a session's energy history, and tests drive the
+   growth branch of the rule from the host; the next advance (single session or batch) uses it. */
+int swl_sim2d_override_peak_energy(swl_sim2d *s, double peak);
 int swl_sim2d_total_steps(const swl_sim2d *s);
 double swl_sim2d_dt(const swl_sim2d *s);
 double swl_sim2d_record_dt(const swl_sim2d *s);
@@ -435,6 +444,9 @@ void swl_sim3d_destroy(swl_sim3d *s);
 int swl_sim3d_advance(swl_sim3d *s, int count);
 int swl_sim3d_reset(swl_sim3d *s);
 int swl_sim3d_step(const swl_sim3d *s);
+/* Replaces the source-window peak energy of the late-growth fault rule (CurvilinearSolver.cpp:477-488:
+   E > 1e4*source_peak after 3/f), as swl_sim2d_override_peak_energy does for 2D. */
+int swl_sim3d_override_source_peak(swl_sim3d *s, double peak);
 int swl_sim3d_total_steps(const swl_sim3d *s);
 double swl_sim3d_dt(const swl_sim3d *s);
 /* layout: nx_pad, ny_pad, nz_pad, receivers, samples recorded so far (t = 0 included) */

@@ -111,6 +111,7 @@ void orc2d_pml(void *h, double *out);
    16..18 deltaP, 19..21 deltaS (all FP64 construction values) */
 int orc2d_model(void *h, int which, double *out);
 int64_t orc2d_pml_slot(int nx_pad, int nz_pad, int width_x, int width_z, int64_t q, int axis);
+void orc2d_set_peak(void *h, double peak); /* test hook: the fault rule's accepted peak energy */
 size_t orc2d_params_size(void);
 
 void *orc3d_create(const orc_params3d *p, char *error, int error_len);
@@ -130,6 +131,7 @@ void orc3d_field(void *h, int f, double *out);
    7 damping, 8 source, 9..11 source gradient */
 int orc3d_model(void *h, int which, double *out);
 size_t orc3d_params_size(void);
+void orc3d_set_source_peak(void *h, double peak); /* test hook: the fault rule's source-window peak */
 
 #ifdef __cplusplus
 }

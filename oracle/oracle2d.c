@@ -957,6 +957,9 @@ int orc2d_reset(void *hh) {
 }
 
 int orc2d_step(void *h) { return ((orc2d *)h)->step; }
+/* Test hook: replaces the accepted peak energy that the late-growth fault rule compares against
+   (CurvilinearSolver2D.cpp:576-578), so a test can drive that branch without an unstable model. */
+void orc2d_set_peak(void *h, double peak) { ((orc2d *)h)->peak = peak; }
 int orc2d_total(void *h) { return ((orc2d *)h)->total; }
 double orc2d_dt(void *h) { return ((orc2d *)h)->dt; }
 double orc2d_record_dt(void *h) {

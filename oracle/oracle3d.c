@@ -930,6 +930,9 @@ int orc3d_advance(void *h, int count, char *e, int el) {
 }
 
 int orc3d_step(void *h) { return ((O3 *)h)->step; }
+/* Test hook: replaces the source-window peak energy that the late-growth fault rule compares against
+   (CurvilinearSolver.cpp:477-488), so a test can drive that branch without an unstable model. */
+void orc3d_set_source_peak(void *h, double peak) { ((O3 *)h)->source_peak = peak; }
 int orc3d_total(void *h) { return ((O3 *)h)->steps; }
 double orc3d_dt(void *h) { return ((O3 *)h)->dt; }
 void orc3d_layout(void *h, int64_t *o) {

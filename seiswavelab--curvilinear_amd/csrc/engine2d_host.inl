@@ -217,8 +217,9 @@ template <class R> struct Engine2D {
     // Per-row material tables when every material array is constant along each row (layered models,
     // grid-conformal by k); the interior then reads one value per row instead of streaming the arrays.
     DeviceBuffer rowTables;
+    int rowsOk = 0; // the tables exist (the per-node arrays are always resident as well)
     void setup_rows(const swl_mesh2d_desc &d) {
-        m.by_row = 0;
+        m.by_row = rowsOk = 0;
         std::vector<const double *> arrays = {d.lambda, d.mu, d.rho};
         for (int l = 0; l < mech; ++l) arrays.push_back(d.delta_p[l]), arrays.push_back(d.delta_s[l]);
         for (const double *a : arrays)
@@ -232,7 +233,7 @@ template <class R> struct Engine2D {
         const R *base = rowTables.as<R>();
         m.lam_k = base, m.mu_k = base + nz, m.rho_k = base + 2 * nz;
         for (int l = 0; l < mech; ++l) m.dP_k[l] = base + (3 + 2 * l) * (size_t)nz, m.dS_k[l] = base + (4 + 2 * l) * (size_t)nz;
-        m.by_row = 1;
+        m.by_row = rowsOk = 1;
     }
 
     // Interior rectangle: depth >= 4, outside every CFS-PML slot, zero sponge damping; x-extent
@@ -646,7 +647,7 @@ int swl_engine2d_kernel_timing(swl_engine2d *e, double *out) {
             out[0] = x.stageMs, out[1] = (double)x.stageLaunches, out[2] = x.finalMs, out[3] = (double)x.finalLaunches;
             out[4] = (double)x.nInterior, out[5] = (double)x.registers;
             out[6] = (double)((x.nx - 6) * (long long)(x.nz - 6)); // nodes updated per stage launch
-            out[7] = (double)x.nGeneral, out[8] = (double)x.m.metric;
+            out[7] = (double)x.nGeneral, out[8] = (double)x.m.metric, out[9] = (double)x.m.by_row;
         });
     });
 }
@@ -657,6 +658,7 @@ int swl_engine2d_set_option(swl_engine2d *e, const char *name, int value) {
             if (key == "nontemporal_stores") x.nt = value != 0;
             else if (key == "stored_metrics") x.m.metric = value == 0 && x.metricOk; // read a, b, J from memory
             else if (key == "lds_tiles") x.tileLds = value != 0;
+            else if (key == "material_rows") x.m.by_row = value != 0 && x.rowsOk; // 0: read the per-node arrays
             else throw std::runtime_error("unknown engine option: " + key);
         });
     });

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <filesystem>
 #include <fstream>
@@ -90,6 +91,12 @@ int swl_sim2d_reset(swl_sim2d *s) {
     });
 }
 int swl_sim2d_step(const swl_sim2d *s) { return s->step; }
+int swl_sim2d_override_peak_energy(swl_sim2d *s, double peak) {
+    return swl::guarded([&] {
+        swl::require(std::isfinite(peak) && peak >= 0, "peak energy must be finite and non-negative");
+        s->peak = peak;
+    });
+}
 int swl_sim2d_total_steps(const swl_sim2d *s) { return s->model.total; }
 double swl_sim2d_dt(const swl_sim2d *s) { return s->model.dt; }
 double swl_sim2d_record_dt(const swl_sim2d *s) { return s->recordDt(); }
@@ -248,9 +255,15 @@ int swl_device_query(int device, swl_device_info *out) {
         hipDeviceProp_t prop{};
         swl::check(hipGetDeviceProperties(&prop, device), "device properties");
         std::strncpy(out->name, prop.name, sizeof out->name - 1);
+        if (!out->name[0] && hipDeviceGetName(out->name, sizeof out->name - 1, device) != hipSuccess) out->name[0] = 0;
         std::strncpy(out->arch, prop.gcnArchName, sizeof out->arch - 1);
         out->total_bytes = prop.totalGlobalMem;
         out->compute_units = prop.multiProcessorCount;
+        if (!out->name[0]) { // no marketing name on this system: name the device by its ISA and size
+            const std::string isa(prop.gcnArchName, std::strcspn(prop.gcnArchName, ":"));
+            std::snprintf(out->name, sizeof out->name, "AMD %s, %d CUs, %.0f GiB", isa.c_str(), prop.multiProcessorCount,
+                          prop.totalGlobalMem / 1073741824.0);
+        }
         swl::DeviceScope scope(device);
         size_t freeBytes = 0, total = 0;
         swl::check(hipMemGetInfo(&freeBytes, &total), "memory query");

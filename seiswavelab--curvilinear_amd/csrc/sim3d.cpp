@@ -58,6 +58,12 @@ int swl_sim3d_reset(swl_sim3d *s) {
     });
 }
 int swl_sim3d_step(const swl_sim3d *s) { return s->step; }
+int swl_sim3d_override_source_peak(swl_sim3d *s, double peak) {
+    return swl::guarded([&] {
+        swl::require(std::isfinite(peak) && peak >= 0, "source-window peak energy must be finite and non-negative");
+        s->sourcePeak = peak;
+    });
+}
 int swl_sim3d_total_steps(const swl_sim3d *s) { return s->model.total; }
 double swl_sim3d_dt(const swl_sim3d *s) { return s->model.dt; }
 int swl_sim3d_layout(const swl_sim3d *s, int64_t *out) {

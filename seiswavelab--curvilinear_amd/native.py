@@ -60,6 +60,7 @@ SIGNATURES = {
     "swl_sim2d_advance": (C.c_int, [vp, C.c_int]),
     "swl_sim2d_reset": (C.c_int, [vp]),
     "swl_sim2d_step": (C.c_int, [vp]),
+    "swl_sim2d_override_peak_energy": (C.c_int, [vp, C.c_double]),
     "swl_sim2d_total_steps": (C.c_int, [vp]),
     "swl_sim2d_dt": (C.c_double, [vp]),
     "swl_sim2d_record_dt": (C.c_double, [vp]),
@@ -144,6 +145,7 @@ SIGNATURES = {
     "swl_sim3d_advance": (C.c_int, [vp, C.c_int]),
     "swl_sim3d_reset": (C.c_int, [vp]),
     "swl_sim3d_step": (C.c_int, [vp]),
+    "swl_sim3d_override_source_peak": (C.c_int, [vp, C.c_double]),
     "swl_sim3d_total_steps": (C.c_int, [vp]),
     "swl_sim3d_dt": (C.c_double, [vp]),
     "swl_sim3d_layout": (C.c_int, [vp, vp]),

@@ -188,16 +188,22 @@ class Simulation:
 
     def kernel_timing(self) -> dict:
         self._need()
-        o = np.zeros(9)
+        o = np.zeros(10)
         native.call("swl_sim2d_kernel_timing", self._h, _dp(o))
         return {"stage_ms": o[0], "stage_launches": int(o[1]), "finalize_ms": o[2], "finalize_launches": int(o[3]),
                 "interior_tiles": int(o[4]), "vgprs": int(o[5]), "nodes_per_launch": int(o[6]),
-                "border_tiles": int(o[7]), "metric_on_the_fly": bool(o[8])}
+                "border_tiles": int(o[7]), "metric_on_the_fly": bool(o[8]), "material_by_row": bool(o[9])}
 
     def fault(self) -> str:
         """The latched fault message ("" if none)."""
         self._need()
         return native.lib().swl_sim2d_fault(self._h).decode()
+
+    def override_peak_energy(self, peak: float):
+        """Replaces the accepted peak energy the late-growth fault rule compares against
+        (CurvilinearSolver2D.cpp:576-578); the next advance, alone or in a batch, uses it."""
+        self._need()
+        native.call("swl_sim2d_override_peak_energy", self._h, float(peak))
 
     def set_option(self, name: str, value: int):
         """Engine tuning switch: "nontemporal_stores" or "stored_metrics"."""
@@ -519,6 +525,11 @@ class Simulation3D:
     def profile(self, enable: bool = True):
         self._need()
         native.call("swl_sim3d_profile", self._h, int(enable))
+
+    def override_source_peak(self, peak: float):
+        """Replaces the source-window peak energy of the late-growth fault rule (CurvilinearSolver.cpp:477-488)."""
+        self._need()
+        native.call("swl_sim3d_override_source_peak", self._h, float(peak))
 
     def kernel_timing(self) -> dict:
         self._need()

@@ -15,6 +15,12 @@ REF_SO = ROOT / "oracle" / "_ref" / "libswlref.so"
 _libs = {}
 
 
+def have_ref() -> bool:
+    """True where the reference build exists (this container only; it never travels to the GPU box).
+    Checks the file without loading it, so merely collecting a test module loads nothing."""
+    return REF_SO.exists()
+
+
 def load(kind: str):
     """kind = 'orc' (oracle) or 'ref' (reference build)."""
     if kind in _libs:
@@ -42,6 +48,10 @@ def load(kind: str):
         if hasattr(lib, pre + "record_dt"):
             getattr(lib, pre + "record_dt").restype = C.c_double
             getattr(lib, pre + "record_dt").argtypes = [C.c_void_p]
+    for name in (f"{kind}2d_set_peak", f"{kind}3d_set_source_peak"):
+        if hasattr(lib, name):
+            getattr(lib, name).restype = None
+            getattr(lib, name).argtypes = [C.c_void_p, C.c_double]
     if hasattr(lib, f"{kind}2d_pml_slot"):
         f = getattr(lib, f"{kind}2d_pml_slot")
         f.restype = C.c_int64
@@ -93,6 +103,10 @@ class CpuSession2D:
 
     def reset(self):
         return bool(self._f("reset")(C.c_void_p(self.h)))
+
+    def set_peak(self, peak):
+        """Oracle test hook: the accepted peak energy of the late-growth fault rule."""
+        self._f("set_peak")(C.c_void_p(self.h), float(peak))
 
     @property
     def step(self):
@@ -215,6 +229,10 @@ class CpuSession3D:
     @property
     def dt(self):
         return self._f("dt")(C.c_void_p(self.h))
+
+    def set_source_peak(self, peak):
+        """Oracle test hook: the source-window peak energy of the late-growth fault rule."""
+        self._f("set_source_peak")(C.c_void_p(self.h), float(peak))
 
     def layout(self):
         o = np.zeros(5, np.int64)

@@ -157,6 +157,7 @@ def test_device_query():
     assert native.device_count() >= 1
     info = native.device_query(0)
     assert info.available and info.total_bytes > 0 and b"gfx9" in info.arch
+    assert info.name and info.compute_units > 0
 
 
 @pytest.mark.parametrize("name", CASES_2D)

@@ -15,7 +15,7 @@ import pathlib
 import numpy as np
 import pytest
 
-from cpu_sessions import CpuSession2D, load, rel_l2
+from cpu_sessions import CpuSession2D, have_ref, load, rel_l2
 from reference_fixtures import FIXTURES_2D, load_fixture, run_oracle_2d
 from seiswavelab_curvilinear_amd.params import (Absorber, CParams2D, GslsSettings, Layer, Parameters, Precision,
                                                  SourceKind, SourceMechanism)
@@ -212,7 +212,7 @@ def ref_case(name):
     return p
 
 
-@pytest.mark.skipif(load("ref") is None, reason="reference build oracle/_ref not present")
+@pytest.mark.skipif(not have_ref(), reason="reference build oracle/_ref not present")
 @pytest.mark.parametrize("name", sorted(REF_CASES))
 @pytest.mark.parametrize("prec", [Precision.Float64, Precision.Float32])
 def test_oracle_bitwise_equals_reference_build(name, prec):
@@ -236,7 +236,7 @@ def test_oracle_bitwise_equals_reference_build(name, prec):
         assert da[k] == db[k] or (math.isinf(da[k]) and math.isinf(db[k])), k
 
 
-@pytest.mark.skipif(load("ref") is None, reason="reference build oracle/_ref not present")
+@pytest.mark.skipif(not have_ref(), reason="reference build oracle/_ref not present")
 def test_pml_slot_map_matches_reference():
     ref, orc = load("ref"), load("orc")
     for nx, nz, wx, wz in ((26, 20, 4, 3), (57, 41, 9, 12), (14, 14, 2, 2)):
@@ -258,3 +258,104 @@ def test_oracle_matches_reference_fixtures(name, prec):
     assert sorted(got) == sorted(want)
     for k in want:
         assert np.array_equal(got[k], want[k]), k
+
+
+# ---- per-node materials (the extension for heterogeneous models, oracle2d.c per-node branch) ----
+# The reference's 2D API has layered models only, so the per-node branch is pinned two ways:
+#  * a layered model restated as per-node arrays (every logical node carries its layer's values) must
+#    reproduce the layered run bit for bit: the reference build's fixtures, its live runs and the
+#    reference's own golden dry-loess trace (GSLS-3, external CFS-PML, curved, layered);
+#  * point anomalies must land on the node whose coordinates (pinned to the reference build) are those
+#    of the logical node, and the ghost and external-strip nodes carry the nearest logical node's values.
+
+
+def pernode_twin(p):
+    """The same model given as per-node Vp, Vs, density on the logical nz x nx grid."""
+    import copy
+    from seiswavelab_curvilinear_amd.workloads import layered_base
+    q = copy.deepcopy(p)
+    if p.layered:
+        rows = layered_base(p.nz, p.spacing, p.layers)
+    else:
+        rows = tuple(np.full(p.nz, v) for v in (p.vp, p.vs, p.density))
+    q.nodeVp, q.nodeVs, q.nodeDensity = (np.repeat(a[:, None], p.nx, axis=1) for a in rows)
+    return q
+
+
+@pytest.mark.parametrize("name,prec", FIXTURES_2D)
+def test_pernode_twin_matches_reference_fixtures(name, prec):
+    p = ref_case(name)
+    p.precision = prec
+    got, want = run_oracle_2d("orc", pernode_twin(p)), load_fixture("2d", name, prec)
+    for k in want:
+        assert np.array_equal(got[k], want[k]), k
+
+
+def test_pernode_twin_of_dry_loess_is_bitwise_the_layered_run():
+    """GSLS-3 + external CFS-PML + curved + two layers, given per node (each node refits its own GSLS
+    material): models, traces, fields and PML memories bitwise equal to the layered run, which
+    test_dry_loess_golden_trace holds to the reference's own trace (600 of its 5664 steps here)."""
+    p = precision_case("dry_loess_gsls3_pml")
+    p.cpuThreads = 8
+    a, b = CpuSession2D("orc", p), CpuSession2D("orc", pernode_twin(p))
+    for which in (3, 4, 5, 12, 13, 16, 17, 18, 19, 20, 21):
+        assert np.array_equal(a.model(which), b.model(which)), which
+    assert a.dt == b.dt and a.total == b.total
+    assert a.advance(600)[0] and b.advance(600)[0]
+    ra, rb = a.receivers(), b.receivers()
+    assert np.array_equal(ra["vx"], rb["vx"]) and np.array_equal(ra["vz"], rb["vz"])
+    for f in range(a.layout()["fields"]):
+        assert np.array_equal(a.field(f), b.field(f)), f
+    assert np.array_equal(a.pml(), b.pml())
+
+
+@pytest.mark.skipif(not have_ref(), reason="reference build oracle/_ref not present")
+@pytest.mark.parametrize("prec", [Precision.Float64, Precision.Float32])
+def test_pernode_twin_equals_reference_build_layered_external(prec):
+    """Layers under an external CFS-PML: the strips below the model take the bottom row's layer in the
+    reference (its depth is clamped to the model's bottom); the per-node branch clamps to the last
+    logical row.  Live against the reference build, every array and trace bit for bit."""
+    p = ref_case("moment_external_pml")
+    p.precision = prec
+    p.layered = True
+    p.layers = [Layer(20, 900, 450, 1800), Layer(30, 1800, 900, 2100), Layer(0, 3000, 1700, 2400)]
+    a, b = CpuSession2D("ref", p), CpuSession2D("orc", pernode_twin(p))
+    assert a.dt == b.dt and a.total == b.total
+    cast = (lambda x: x.astype(np.float32).astype(np.float64)) if prec == Precision.Float32 else (lambda x: x)
+    for which in (5, 12, 13):  # the reference build exposes density, Vp and Vs
+        assert np.array_equal(a.model(which), cast(b.model(which))), which
+    assert a.advance(a.total)[0] and b.advance(b.total)[0]
+    ra, rb = a.receivers(), b.receivers()
+    assert np.array_equal(ra["vx"], rb["vx"]) and np.array_equal(ra["vz"], rb["vz"])
+    for f in range(a.layout()["fields"]):
+        assert np.array_equal(a.field(f), b.field(f)), f
+    assert np.array_equal(a.pml(), b.pml())
+
+
+@pytest.mark.parametrize("external", [True, False])
+def test_pernode_index_map_follows_coordinates(external):
+    """Point anomalies at logical nodes (corners, edges, interior) of a flat model.  The padded node that
+    carries each is found from the coordinate arrays (x = i h, depth = k h on a flat grid, pinned to the
+    reference build), not from the index formula; ghost nodes and external-strip nodes must carry the
+    values of the nearest logical node."""
+    nx, nz, h = 41, 33, 2.0
+    p = Parameters(nx=nx, nz=nz, spacing=h, relief=0, vp=3000, vs=1700, density=2400, sourceZ=10,
+                   receiverCount=4, absorber=Absorber.CfsPml, spongeX=7, spongeZ=6, externalAbsorption=external)
+    vp, vs, rho = np.full((nz, nx), 3000.0), np.full((nz, nx), 1700.0), np.full((nz, nx), 2400.0)
+    spots = [(0, 0), (nx - 1, 0), (0, nz - 1), (nx - 1, nz - 1), (17, 11), (0, 20), (25, nz - 1), (nx - 1, 5), (9, 0)]
+    for n, (x, z) in enumerate(spots):
+        vp[z, x], vs[z, x], rho[z, x] = 3100.0 + 10 * n, 1750.0 + 5 * n, 2450.0 + 3 * n
+    p.nodeVp, p.nodeVs, p.nodeDensity = vp, vs, rho
+    s = CpuSession2D("orc", p)
+    L = s.layout()
+    coord = s.model(11).reshape(L["nz"], L["nx"], 2)
+    li = np.clip(np.rint(coord[..., 0] / h).astype(int), 0, nx - 1)
+    lk = np.clip(np.rint(coord[..., 1] / h).astype(int), 0, nz - 1)
+    assert L["offsetX"] == (7 if external else 0)
+    for which, want in ((12, vp), (13, vs), (5, rho)):
+        got = s.model(which).reshape(L["nz"], L["nx"])
+        assert np.array_equal(got, want[lk, li]), which
+    # each interior anomaly sits at exactly one padded node, the one whose coordinates are its own
+    got = s.model(12).reshape(L["nz"], L["nx"])
+    k, i = np.nonzero(got == vp[11, 17])
+    assert len(k) == 1 and coord[k[0], i[0], 0] == 17 * h and coord[k[0], i[0], 1] == 11 * h

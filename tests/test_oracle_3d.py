@@ -7,7 +7,7 @@ the build is present and against its committed outputs (tests/golden/reference_3
 import numpy as np
 import pytest
 
-from cpu_sessions import CpuSession3D, load
+from cpu_sessions import CpuSession3D, have_ref, load
 from reference_fixtures import load_fixture, run_oracle_3d
 from seiswavelab_curvilinear_amd.params import (LayerDepthMode3D, MaterialLayer3D, ReceiverArrayKind3D,
                                                 SimulationParameters, SourceKind, SourceMechanism, SurfaceGrid3D)
@@ -71,7 +71,7 @@ def case(name):
 
 
 CASES = ["analytic_sponge_force", "dem_layers_moment", "dem_gridindex_layers", "flat_datum_layers"]
-HAVE_REF = load("ref") is not None and hasattr(load("ref"), "ref3d_create")
+HAVE_REF = have_ref()
 
 
 @pytest.mark.skipif(not HAVE_REF, reason="reference build oracle/_ref not present")

@@ -14,11 +14,11 @@ import struct
 import numpy as np
 import pytest
 
-from cpu_sessions import load
+from cpu_sessions import have_ref, load
 from reference_fixtures import SEGY_VARIANTS, segy_fixture_path, segy_records
 from seiswavelab_curvilinear_amd.records import SegyExportProfile, TraceRecord, read_segy, write_segy
 
-HAVE_REF = load("ref") is not None and hasattr(load("ref"), "ref_segy_write")
+HAVE_REF = have_ref()
 
 
 def traces():
