@@ -1,0 +1,84 @@
+"""The engine seam itself (GPU): the HIP engine that replaces CudaEngine, created from the oracle's FP64 mesh
+(the restatement of the reference's Point::MeshT) and stepped the way the reference's advance() steps
+CudaEngine (src/two_d/CurvilinearSolver2D.cpp:553-597): step(step*dt, dt, f, amplitude/h^2, step), then
+diagnostics, the fault rule and sample, once per step.  No session code is involved.
+
+Bars as for the sessions: FP64 <= 1e-10 relative L2 against the oracle (fields, PML memories, traces),
+energy <= 1e-12; FP32 <= 1e-4.
+"""
+import numpy as np
+import pytest
+
+from cpu_sessions import CpuSession2D, rel_l2
+from seiswavelab_curvilinear_amd import Precision
+from seiswavelab_curvilinear_amd.engine import Engine2D, mesh_from_arrays, source_scale
+from test_oracle_2d import ref_case
+
+pytestmark = pytest.mark.gpu
+
+
+def engine_for(name, precision):
+    p = ref_case(name)
+    p.precision = Precision.Float64
+    o = CpuSession2D("orc", p)
+    mesh = mesh_from_arrays(o.model, o.layout(), p, precision=precision)
+    return p, o, Engine2D(mesh, o.probe_indices(), 0)
+
+
+def advance(e, p, dt, steps):
+    """The reference's advance() loop over CudaEngine; returns the per-step raw samples and diagnostics."""
+    peak, samples, energies = 0.0, [], []
+    scale = source_scale(p)
+    for n in range(steps):
+        e.step(n * dt, dt, p.frequency, scale, n)
+        d = e.diagnostics()
+        assert d["finite"] and not ((n + 1) * dt > 3 / p.frequency and peak > 0 and d["energy"] > 1e4 * peak)
+        samples.append(e.sample())
+        energies.append(d["energy"])
+        peak = max(peak, d["energy"])
+    return np.array(samples), np.array(energies)
+
+
+@pytest.mark.parametrize("name", ["sine_terrain_small", "moment_external_pml"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_engine_from_oracle_mesh_matches_oracle(name, precision):
+    p, o, e = engine_for(name, precision)
+    print(f"[engine seam] {name} precision {precision}", flush=True)
+    steps = o.total
+    raw, energy = advance(e, p, o.dt, steps)
+    energy_o = []
+    for _ in range(steps):
+        assert o.advance(1)[0]
+        energy_o.append(o.diagnostics()["energy"])
+    rec = o.receivers()  # sampleIntervalMs = 0: the record after t = 0 is the raw probe values
+    bar = 1e-10 if precision == 0 else 1e-4
+    got = np.concatenate([raw[:, :, 0].T.ravel(), raw[:, :, 1].T.ravel()])
+    want = np.concatenate([rec["vx"][:, 1:].ravel(), rec["vz"][:, 1:].ravel()])
+    assert np.abs(want).max() > 0
+    assert rel_l2(want, got) <= bar
+    for f in range(5):
+        assert rel_l2(o.field(f), e.snapshot(f)) <= bar, f
+    if e.cpml:
+        assert rel_l2(o.pml(), e.pml()) <= bar
+    np.testing.assert_allclose(energy, energy_o, rtol=1e-12 if precision == 0 else 1e-4, atol=0)
+    e.close()
+
+
+def test_engine_rollback_and_reset():
+    """CudaEngine::rollback returns to the state before the last step; reset clears every field."""
+    p, o, e = engine_for("sponge_tilted_force", 0)
+    scale = source_scale(p)
+    for n in range(20):
+        e.step(n * o.dt, o.dt, p.frequency, scale, n)
+    before = [e.snapshot(f) for f in range(5)]
+    e.step(20 * o.dt, o.dt, p.frequency, scale, 20)
+    assert not np.array_equal(e.snapshot(0), before[0])
+    e.rollback()
+    for f in range(5):
+        assert np.array_equal(e.snapshot(f), before[f]), f
+    assert o.advance(20)[0]
+    for f in range(5):
+        assert rel_l2(o.field(f), before[f]) <= 1e-10, f
+    e.reset()
+    assert all(not np.any(e.snapshot(f)) for f in range(5))
+    assert e.residentBytes() > 0
