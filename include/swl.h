@@ -155,6 +155,13 @@ int swl_engine2d_rollback(swl_engine2d *e);
 int swl_engine2d_reset(swl_engine2d *e);
 /* CudaEngine::diagnostics, CudaEngine2D.cu:155-158. */
 int swl_engine2d_diagnostics(swl_engine2d *e, swl_diag2d *out);
+/* No counterpart in the reference.  swl_engine2d_diagnostics for callers that cannot allocate device memory
+   (code between HIP graph captures, tight serving loops, hosts that forbid hipMalloc): the same values, with
+   the per-tile partials of the inspection kernel in a fixed 4 MiB device array (one per GPU, held until the
+   process ends) instead of a per-call allocation.  Grids of more than 131072 tiles of 64x4 computational
+   nodes (about 33.5 M nodes) are rejected with an error.  Calls on one GPU share that array and take turns
+   (a per-device lock in this process); the call waits for the engine's stream. */
+int swl_engine2d_diagnostics_noalloc(swl_engine2d *e, swl_diag2d *out);
 /* CudaEngine::sample, CudaEngine2D.cu:159-162: out[2r] = Vx, out[2r+1] = Vz. */
 int swl_engine2d_sample(swl_engine2d *e, double *out);
 /* CudaEngine::frame, CudaEngine2D.cu:163-165: component 0 Vz, 1 Vx, 2 |V|, 3 pressure. */

@@ -2,8 +2,39 @@
 // Owns every device allocation, converts between the reference's padded storage (i + nx*k) and
 // the engine's aligned layout (off + i + ld*k), classifies the grid into the interior rectangle and
 // the general border, and enqueues the stage / inspection kernels on the engine's streams.
+
+// Fixed device scratch of swl_engine2d_diagnostics_noalloc: the per-tile partials of stats_kernel
+// (32 B each), so that entry point calls no allocator.  Each GPU holds its own copy until the process
+// ends.  In host code this name is not a device address: kernels get the address that
+// hipGetSymbolAddress returns for the current device (swl::scratch_slot).
+static __device__ __align__(256) unsigned char scratch[size_t(4) << 20];
+
 namespace swl {
 namespace {
+
+constexpr size_t kScratchBytes = sizeof(scratch); // 131072 partials: grids of up to 131072 tiles of 64x4 nodes
+
+// Device address of `scratch` on one GPU, looked up once per device with that device current, and the
+// mutex that makes calls on that GPU take turns (they share its one copy of the array).
+struct ScratchSlot {
+    std::mutex use;
+    void *ptr = nullptr;
+};
+ScratchSlot &scratch_slot(int device) {
+    static std::mutex lookup;
+    static std::vector<std::unique_ptr<ScratchSlot>> slots;
+    std::lock_guard<std::mutex> guard(lookup);
+    require(device >= 0, "invalid device");
+    if ((size_t)device >= slots.size()) slots.resize((size_t)device + 1);
+    if (!slots[device]) {
+        auto slot = std::make_unique<ScratchSlot>();
+        DeviceScope scope(device);
+        check(hipGetSymbolAddress(&slot->ptr, HIP_SYMBOL(scratch)), "scratch address");
+        require(slot->ptr != nullptr, "scratch address");
+        slots[device] = std::move(slot);
+    }
+    return *slots[device];
+}
 
 inline uint64_t next_engine_uid() {
     static std::atomic<uint64_t> counter{0};
@@ -396,10 +427,28 @@ template <class R> struct Engine2D {
         DeviceScope scope(device);
         DeviceBuffer tmp;
         tmp.allocate((size_t)ntiles * sizeof(Partial));
-        stats_kernel<R><<<ntiles, THREADS, 0, stream>>>(m, S(cur), cpml ? P(cur) : nullptr, count, tmp.as<Partial>(), ntx,
+        diagnostics_with(tmp.as<Partial>(), out);
+    }
+    // The same with the partials in the GPU's fixed scratch array: no allocation.  Every tile's block writes
+    // its partial, so nothing needs clearing, and only this engine's ntiles partials are read back.
+    void diagnostics_noalloc(swl_diag2d &out) {
+        DeviceScope scope(device);
+        const size_t need = (size_t)ntiles * sizeof(Partial);
+        if (need > kScratchBytes)
+            throw std::runtime_error("no-allocation diagnostics: this 2D grid has " + std::to_string(ntiles) +
+                                     " tiles, whose partials need " + std::to_string(need) + " bytes; the fixed scratch holds " +
+                                     std::to_string(kScratchBytes) + " bytes (" + std::to_string(kScratchBytes / sizeof(Partial)) +
+                                     " tiles of 64x4 nodes)");
+        ScratchSlot &slot = scratch_slot(device);
+        std::lock_guard<std::mutex> use(slot.use);
+        diagnostics_with(static_cast<Partial *>(slot.ptr), out);
+    }
+    // stats_kernel over the accepted state into `partials` (at least ntiles entries), reduced on the host
+    void diagnostics_with(Partial *partials, swl_diag2d &out) {
+        stats_kernel<R><<<ntiles, THREADS, 0, stream>>>(m, S(cur), cpml ? P(cur) : nullptr, count, partials, ntx,
                                                           ntiles);
         check(hipGetLastError(), "2D diagnostics");
-        check(hipMemcpyAsync(host_partials.data(), tmp.ptr, ntiles * sizeof(Partial), hipMemcpyDeviceToHost, stream),
+        check(hipMemcpyAsync(host_partials.data(), partials, ntiles * sizeof(Partial), hipMemcpyDeviceToHost, stream),
               "2D diagnostics readback");
         check(hipStreamSynchronize(stream), "2D diagnostics");
         out.energy = out.max_velocity = out.traction_relative = 0;
@@ -611,6 +660,9 @@ int swl_engine2d_reset(swl_engine2d *e) {
 }
 int swl_engine2d_diagnostics(swl_engine2d *e, swl_diag2d *out) {
     return swl::guarded([&] { e->with([&](auto &x) { x.diagnostics(*out); }); });
+}
+int swl_engine2d_diagnostics_noalloc(swl_engine2d *e, swl_diag2d *out) {
+    return swl::guarded([&] { e->with([&](auto &x) { x.diagnostics_noalloc(*out); }); });
 }
 int swl_engine2d_sample(swl_engine2d *e, double *out) {
     return swl::guarded([&] { e->with([&](auto &x) { x.sample(out); }); });

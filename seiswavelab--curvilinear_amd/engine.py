@@ -119,6 +119,13 @@ class Engine2D:
         native.call("swl_engine2d_diagnostics", self._h, C.byref(o))
         return self._diag(o)
 
+    def diagnostics_noalloc(self) -> dict:
+        """diagnostics() without any device allocation: the inspection's per-tile partials go to a fixed
+        device array (no counterpart in the reference; for callers that cannot allocate)."""
+        o = Diag2D()
+        native.call("swl_engine2d_diagnostics_noalloc", self._h, C.byref(o))
+        return self._diag(o)
+
     def sample(self) -> np.ndarray:
         """CudaEngine::sample (CudaEngine2D.cu:159-162): (receivers, 2) of Vx, Vz."""
         o = np.zeros(2 * max(self.receivers, 1))

@@ -39,6 +39,7 @@ SIGNATURES = {
     "swl_engine2d_rollback": (C.c_int, [vp]),
     "swl_engine2d_reset": (C.c_int, [vp]),
     "swl_engine2d_diagnostics": (C.c_int, [vp, vp]),
+    "swl_engine2d_diagnostics_noalloc": (C.c_int, [vp, vp]),
     "swl_engine2d_sample": (C.c_int, [vp, vp]),
     "swl_engine2d_frame": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_int, C.c_int]),
     "swl_engine2d_snapshot": (C.c_int, [vp, C.c_int, vp]),
