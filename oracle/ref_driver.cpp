@@ -393,6 +393,13 @@ int ref2d_write_snapshot(void *h, const char *path, const int *components, int i
     copyError(e, error, len);
     return ok;
 }
+// writeReceivers (include/Seis2D/CurvilinearSolver2D.hpp:140): the receiver CSV with its provenance header
+int ref2d_write_receivers(void *h, const char *path, char *error, int len) {
+    std::string e;
+    const bool ok = S2::writeReceivers(*static_cast<S2::Simulation *>(h), path, e);
+    copyError(e, error, len);
+    return ok;
+}
 } // extern "C"
 
 // ---- 3D receiver CSV and wavefield VTS: the reference's own writers (include/Seis3D/ReceiverIo.hpp,

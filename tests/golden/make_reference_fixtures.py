@@ -20,10 +20,11 @@ HERE = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(HERE.parent))
 import conftest  # noqa: E402,F401  registers the package
 from cpu_sessions import load  # noqa: E402
-from reference_fixtures import (FIXTURES_2D, RECEIVERS3D_EXPORTS, SEGY_VARIANTS, VTS3D_CASES,  # noqa: E402
-                                VTS_ALL_COMPONENTS, VTS_MODEL_VARIANTS, VTS_SNAPSHOT_CASES, fixture_path,
-                                receivers3d_fixture_path, run_oracle_2d, run_oracle_3d, segy_fixture_path,
-                                segy_records, vts3d_fixture_path, vts_case, vts_fixture_path)
+from reference_fixtures import (FIXTURES_2D, FULL_2D, RECEIVERS2D_CASES, RECEIVERS3D_EXPORTS,  # noqa: E402
+                                SEGY_VARIANTS, VTS3D_CASES, VTS_ALL_COMPONENTS, VTS_MODEL_VARIANTS,
+                                VTS_SNAPSHOT_CASES, fixture_path, receivers2d_fixture_path, receivers3d_fixture_path,
+                                reduce_full_2d, run_oracle_2d, run_oracle_3d, segy_fixture_path, segy_records,
+                                vts3d_fixture_path, vts_case, vts_fixture_path)
 from test_oracle_2d import ref_case  # noqa: E402
 from test_oracle_3d import CASES, case  # noqa: E402
 
@@ -110,6 +111,35 @@ def write_export3d_fixtures():
             s.close()
 
 
+def write_receivers2d_fixtures():
+    """The reference's writeReceivers after each RECEIVERS2D_CASES case's full run (reference CPU session)."""
+    import ctypes as C
+    import gzip
+    import shutil
+    import tempfile
+    from cpu_sessions import CpuSession2D
+    lib = load("ref")
+    lib.ref2d_write_receivers.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int]
+    err = C.create_string_buffer(512)
+    with tempfile.TemporaryDirectory() as tmp:
+        for name in RECEIVERS2D_CASES:
+            s = CpuSession2D("ref", ref_case(name))
+            ok, e = s.advance(s.total)
+            assert ok, e
+            assert lib.ref2d_write_receivers(C.c_void_p(s.h), f"{tmp}/r.csv".encode(), err, 512), err.value
+            with open(f"{tmp}/r.csv", "rb") as a, gzip.GzipFile(receivers2d_fixture_path(name), "wb", mtime=0) as b:
+                shutil.copyfileobj(a, b)
+            s.close()
+
+
+def write_full_2d_fixture():
+    """The 500x200 sine-terrain workload (seed 1) in FP64 for all its 2000 steps on the reference build."""
+    from seiswavelab_curvilinear_amd import workloads
+    p = workloads.sine_terrain_500x200(seed=1)
+    p.cpuThreads = 8
+    np.savez_compressed(fixture_path("2d", FULL_2D), **reduce_full_2d(run_oracle_2d("ref", p)))
+
+
 def main():
     if load("ref") is None:
         sys.exit("oracle/_ref/libswlref.so is missing: build it with `make -C oracle ref` first")
@@ -122,6 +152,8 @@ def main():
     write_segy_fixtures()
     write_vts_fixtures()
     write_export3d_fixtures()
+    write_receivers2d_fixtures()
+    write_full_2d_fixture()
     for f in sorted(list(HERE.glob("reference_*.npz")) + list(HERE.glob("segy_*.sgy")) + list(HERE.glob("*.gz"))):
         print(f.name, f.stat().st_size)
 

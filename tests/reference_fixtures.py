@@ -50,6 +50,30 @@ def run_oracle_2d(kind, p):
     return out
 
 
+# ---- the 500x200 sine-terrain workload at full size (seed 1, FP64, all 2000 steps), run by the reference build
+FULL_2D = "sine_terrain_500x200"
+FULL_2D_TRACE_STRIDE = 10  # full traces of every 10th receiver are kept; the L2 norm of every receiver's traces
+
+
+def reduce_full_2d(run):
+    """The stored form of a full-size run: every field of run_oracle_2d except the traces, which are kept in full
+    for every FULL_2D_TRACE_STRIDE-th receiver and as per-receiver L2 norms (Vx, Vz) for all."""
+    out = {k: v for k, v in run.items() if k not in ("vx", "vz")}
+    out["vx_kept"] = run["vx"][::FULL_2D_TRACE_STRIDE]
+    out["vz_kept"] = run["vz"][::FULL_2D_TRACE_STRIDE]
+    out["trace_norms"] = np.stack([np.sqrt(np.sum(run["vx"] ** 2, axis=1)), np.sqrt(np.sum(run["vz"] ** 2, axis=1))], 1)
+    return out
+
+
+# ---- 2D receiver CSV (writeReceivers, CurvilinearSolver2D.cpp:811-854) written by the reference build after
+#      the case's full run
+RECEIVERS2D_CASES = ["sponge_tilted_force", "moment_external_pml"]
+
+
+def receivers2d_fixture_path(name):
+    return GOLDEN / f"receivers2d_{name}.csv.gz"
+
+
 def run_oracle_3d(kind, p):
     from cpu_sessions import CpuSession3D
     s = CpuSession3D(kind, p)

@@ -296,3 +296,35 @@ def test_lds_tiles_equal_direct_loads(prec):
     assert np.array_equal(ra["vx"], rb["vx"]) and np.array_equal(ra["vz"], rb["vz"])
     for f in range(5):
         assert np.array_equal(fa[f], fb[f]), f
+
+
+@pytest.mark.parametrize("prec", [Precision.Float64, Precision.Float32])
+def test_full_size_sine_terrain_matches_reference_build(prec):
+    """The 500x200 sine-terrain workload (seed 1) for all its 2000 steps against the reference build's FP64 run:
+    dt and step count exactly, kept traces and every receiver's trace norms, the energy after every step and
+    the final field statistics; FP64 <= 1e-10 (energy 1e-12), FP32 <= 1e-4."""
+    from reference_fixtures import FULL_2D, FULL_2D_TRACE_STRIDE
+    from seiswavelab_curvilinear_amd import workloads
+    want = load_fixture("2d", FULL_2D)
+    p = workloads.sine_terrain_500x200(seed=1, precision=prec)
+    print(f"[hip2d full size] 500x200x2000 {prec.name}", flush=True)
+    s = Simulation()
+    ok, err = s.initialize(p, BackendKind.Hip, 0)
+    assert ok, err
+    assert s.dt() == float(want["dt"]) and s.totalSteps() == int(want["total"]) == 2000
+    energy, peak, stats = record_2d(s, s.totalSteps(),
+                                    lambda x: (x.diagnostics()["energy"], x.diagnostics()["peakEnergy"]),
+                                    lambda x: [x.snapshot_field(f) for f in range(5)])
+    r = s.receiver_arrays()
+    assert r["vx"].shape == (500, 2001)
+    got = np.concatenate([r["vx"][::FULL_2D_TRACE_STRIDE].ravel(), r["vz"][::FULL_2D_TRACE_STRIDE].ravel()])
+    kept = np.concatenate([want["vx_kept"].ravel(), want["vz_kept"].ravel()])
+    norms = np.stack([np.sqrt(np.sum(r["vx"] ** 2, axis=1)), np.sqrt(np.sum(r["vz"] ** 2, axis=1))], 1)
+    bar = FP64_TOL if prec == Precision.Float64 else FP32_GATHER_TOL
+    assert rel_l2(kept, got) <= bar
+    assert rel_l2(want["trace_norms"], norms) <= bar
+    np.testing.assert_allclose(energy, want["energy"], rtol=1e-12 if prec == Precision.Float64 else 1e-4,
+                               atol=0 if prec == Precision.Float64 else 1e-4 * want["energy"].max())
+    for f in range(5):  # L2 norm, sum, max |value|; the sum is held to the norm's scale
+        np.testing.assert_allclose(stats[f], want["field_stats"][f], rtol=bar, atol=bar * want["field_stats"][f][0])
+    s.close()

@@ -359,3 +359,18 @@ def test_pernode_index_map_follows_coordinates(external):
     got = s.model(12).reshape(L["nz"], L["nx"])
     k, i = np.nonzero(got == vp[11, 17])
     assert len(k) == 1 and coord[k[0], i[0], 0] == 17 * h and coord[k[0], i[0], 1] == 11 * h
+
+
+def test_oracle_full_size_sine_terrain_matches_reference_build():
+    """The 500x200 sine-terrain workload (seed 1: three layers, imported PCHIP surface at every node, 30 Hz
+    surface force, 20-node CFS-PML, CFL 0.5) in FP64 for all its 2000 steps, bit for bit against the
+    reference build's run (tests/golden/reference_2d_sine_terrain_500x200_f64.npz)."""
+    from reference_fixtures import FULL_2D, reduce_full_2d
+    from seiswavelab_curvilinear_amd import workloads
+    p = workloads.sine_terrain_500x200(seed=1)
+    p.cpuThreads = 8
+    got, want = reduce_full_2d(run_oracle_2d("orc", p)), load_fixture("2d", FULL_2D)
+    assert int(want["total"]) == 2000
+    assert sorted(got) == sorted(want)
+    for k in want:
+        assert np.array_equal(got[k], want[k]), k
