@@ -63,7 +63,9 @@ typedef struct swl_params2d {
     const double *node_vp, *node_vs, *node_density;
     int threads;
     /* receiver policy of the interactive survey (Parameters::autoSurfaceReceivers / receiverTargetCount,
-       CurvilinearSolver2D.hpp:45-46): stored and saved with the model; automatic layout is not provided */
+       CurvilinearSolver2D.hpp:45-46): with auto_surface_receivers the session lays the line out itself
+       (synchronizeSurfaceReceivers, include/Seis2D/SurfaceSurvey2D.hpp:27-44) and reports it back through
+       swl_sim2d_parameters */
     int auto_surface_receivers, receiver_target_count;
 } swl_params2d;
 
@@ -223,6 +225,10 @@ void swl_sim2d_destroy(swl_sim2d *s);
 int swl_sim2d_advance(swl_sim2d *s, int count);
 int swl_sim2d_reset(swl_sim2d *s);
 int swl_sim2d_step(const swl_sim2d *s);
+/* Simulation::parameters() (CurvilinearSolver2D.hpp:106): the session's logical parameters, with the automatic
+   surface receiver layout applied (CurvilinearSolver2D.cpp:164-168); surface_xz points into the session, the
+   per-node arrays are not echoed (null). */
+int swl_sim2d_parameters(const swl_sim2d *s, swl_params2d *out);
 /* Replaces the accepted peak energy that the late-growth fault rule compares against
    (!finite || (t > 3/f && peak > 0 && E > 1e4*peak), CurvilinearSolver2D.cpp:576-578, where the
    reference keeps it private).  Lets a caller restore a session's energy history, and tests drive the
@@ -254,6 +260,12 @@ int swl_sim2d_read_probes(const swl_sim2d *s, const int32_t *xz, int count, doub
 /* writeReceivers (CurvilinearSolver2D.cpp:811-854): receiver CSV with provenance header. */
 int swl_sim2d_write_receivers(const swl_sim2d *s, const char *path);
 int64_t swl_pml_slot2d(int nx_pad, int nz_pad, int width_x, int width_z, int64_t q, int axis);
+/* Detail::memoryBudget (src/two_d/MemoryBudget2D.hpp:12-38): out[0] = host bytes, out[1] = device bytes a session
+   adds, for backend 0 (the reference's CPU path) and 1 (its CUDA engine) as the reference computes them, and for
+   2 (this library's HIP engine: host bytes as for CUDA, device bytes an upper bound of the engine's allocations).
+   swl_sim2d_create refuses a session whose HIP budget exceeds 80 % of the free host memory or 85 % of the
+   device's free memory, with the reference's message (CurvilinearSolver2D.cpp:247-266). */
+int swl_memory_budget2d(const swl_params2d *p, int backend, double *out);
 size_t swl_params2d_size(void);
 size_t swl_params3d_size(void);
 
@@ -447,6 +459,11 @@ typedef struct swl_sim3d swl_sim3d;
 int swl_sim3d_create(const swl_params3d *p, int backend, int device, int precision, swl_sim3d **out);
 /* FP64 model only, no device (for checking the construction). */
 int swl_sim3d_plan(const swl_params3d *p, swl_sim3d **out);
+/* estimateModelMemory3D (include/Seis3D/ModelPresets3D.hpp:38-49): out[0] padded points, out[1] CPU MiB,
+   out[2] CUDA device MiB, out[3] CUDA host MiB, out[4] within the padded-node budget (0/1); out[5] this library's
+   HIP engine device bytes at `precision` (an upper bound).  swl_sim3d_create applies fitsMemory3D
+   (ModelPresets3D.hpp:50-56) to the host estimate and the HIP device bytes (90 % of the device's free memory). */
+int swl_memory_estimate3d(const swl_params3d *p, int precision, double *out);
 void swl_sim3d_destroy(swl_sim3d *s);
 int swl_sim3d_advance(swl_sim3d *s, int count);
 int swl_sim3d_reset(swl_sim3d *s);

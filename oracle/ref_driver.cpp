@@ -8,7 +8,9 @@
 #include "Seis2D/CurvilinearSolver2D.hpp"
 #include "Seis3D/AbsorbingBoundary3D.hpp"
 #include "Seis3D/CurvilinearSolver.hpp"
+#include "Seis3D/ModelPresets3D.hpp"
 #include "two_d/CfsPml2D.hpp"
+#include "two_d/MemoryBudget2D.hpp"
 #include "two_d/PointKernels2D.hpp"
 #include <algorithm>
 #include <cstring>
@@ -54,6 +56,38 @@ S2::Parameters to2d(const orc_params2d &o) {
     p.importedSurface = o.imported_surface, p.surfaceDatum = o.surface_datum;
     p.surfaceSamples.clear();
     for (int i = 0; i < o.n_surface; ++i) p.surfaceSamples.push_back({o.surface_xz[2 * i], o.surface_xz[2 * i + 1]});
+    return p;
+}
+S3::SimulationParameters to3d(const orc_params3d *o) {
+    S3::SimulationParameters p;
+    p.nx = o->nx, p.ny = o->ny, p.nz = o->nz, p.spacing = o->spacing, p.vp = o->vp, p.vs = o->vs, p.density = o->density;
+    p.layeredMaterial = o->layered != 0;
+    p.layerDepthMode = static_cast<S3::LayerDepthMode3D>(o->layer_depth_mode);
+    p.layers.clear();
+    for (int l = 0; l < o->n_layers; ++l)
+        p.layers.push_back({"layer", o->layer_thickness[l], o->layer_vp[l], o->layer_vs[l], o->layer_density[l]});
+    p.dominantFrequency = o->dominant_frequency;
+    p.sourceX = o->source_x, p.sourceY = o->source_y, p.sourceZ = o->source_z;
+    p.source.kind = o->source_kind ? SeisWave::SourceKind::IsotropicMoment : SeisWave::SourceKind::DirectionalForce;
+    p.source.direction = {o->source_direction[0], o->source_direction[1], o->source_direction[2]};
+    p.sourceForce = o->source_force, p.sourceMoment = o->source_moment;
+    p.duration = o->duration, p.maxSteps = o->max_steps, p.cflSafety = o->cfl;
+    p.topographyAmplitude = o->topography_amplitude, p.topographyWavelength = o->topography_wavelength;
+    p.terrainTransitionDepth = o->terrain_transition_depth;
+    if (o->dem_nx > 0 && o->dem_elevations) {
+        auto &g = p.surfaceGrid;
+        g.nx = o->dem_nx, g.ny = o->dem_ny, g.spacing = o->dem_spacing, g.originX = o->dem_origin_x;
+        g.originY = o->dem_origin_y, g.datum = o->dem_datum;
+        g.elevations.assign(o->dem_elevations, o->dem_elevations + static_cast<size_t>(o->dem_nx) * o->dem_ny);
+    }
+    auto &r = p.receiverArray;
+    r.kind = static_cast<S3::ReceiverArrayKind3D>(o->receiver_kind);
+    r.startX = o->receiver_start_x, r.startY = o->receiver_start_y, r.z = o->receiver_z;
+    r.countX = o->receiver_count_x, r.countY = o->receiver_count_y, r.strideX = o->receiver_stride_x;
+    r.strideY = o->receiver_stride_y;
+    p.absorbingBoundary.enabled = o->absorbing != 0;
+    p.absorbingBoundary.cells = {o->absorbing_cells[0], o->absorbing_cells[1], o->absorbing_cells[2]};
+    p.absorbingBoundary.strength = o->absorbing_strength;
     return p;
 }
 } // namespace
@@ -160,35 +194,7 @@ int64_t ref2d_pml_slot(int nx_pad, int nz_pad, int width_x, int width_z, int64_t
 
 // ---- 3D: CpuSimulation3D on the CPU backend ----
 void *ref3d_create(const orc_params3d *o, char *error, int len) {
-    S3::SimulationParameters p;
-    p.nx = o->nx, p.ny = o->ny, p.nz = o->nz, p.spacing = o->spacing, p.vp = o->vp, p.vs = o->vs, p.density = o->density;
-    p.layeredMaterial = o->layered != 0;
-    p.layerDepthMode = static_cast<S3::LayerDepthMode3D>(o->layer_depth_mode);
-    p.layers.clear();
-    for (int l = 0; l < o->n_layers; ++l)
-        p.layers.push_back({"layer", o->layer_thickness[l], o->layer_vp[l], o->layer_vs[l], o->layer_density[l]});
-    p.dominantFrequency = o->dominant_frequency;
-    p.sourceX = o->source_x, p.sourceY = o->source_y, p.sourceZ = o->source_z;
-    p.source.kind = o->source_kind ? SeisWave::SourceKind::IsotropicMoment : SeisWave::SourceKind::DirectionalForce;
-    p.source.direction = {o->source_direction[0], o->source_direction[1], o->source_direction[2]};
-    p.sourceForce = o->source_force, p.sourceMoment = o->source_moment;
-    p.duration = o->duration, p.maxSteps = o->max_steps, p.cflSafety = o->cfl;
-    p.topographyAmplitude = o->topography_amplitude, p.topographyWavelength = o->topography_wavelength;
-    p.terrainTransitionDepth = o->terrain_transition_depth;
-    if (o->dem_nx > 0 && o->dem_elevations) {
-        auto &g = p.surfaceGrid;
-        g.nx = o->dem_nx, g.ny = o->dem_ny, g.spacing = o->dem_spacing, g.originX = o->dem_origin_x;
-        g.originY = o->dem_origin_y, g.datum = o->dem_datum;
-        g.elevations.assign(o->dem_elevations, o->dem_elevations + static_cast<size_t>(o->dem_nx) * o->dem_ny);
-    }
-    auto &r = p.receiverArray;
-    r.kind = static_cast<S3::ReceiverArrayKind3D>(o->receiver_kind);
-    r.startX = o->receiver_start_x, r.startY = o->receiver_start_y, r.z = o->receiver_z;
-    r.countX = o->receiver_count_x, r.countY = o->receiver_count_y, r.strideX = o->receiver_stride_x;
-    r.strideY = o->receiver_stride_y;
-    p.absorbingBoundary.enabled = o->absorbing != 0;
-    p.absorbingBoundary.cells = {o->absorbing_cells[0], o->absorbing_cells[1], o->absorbing_cells[2]};
-    p.absorbingBoundary.strength = o->absorbing_strength;
+    const S3::SimulationParameters p = to3d(o);
     auto s = std::make_unique<S3::CpuSimulation3D>(S3::BackendKind3D::Cpu);
     std::string e;
     if (!s->initialize(p, e, S3::ComputeSettings3D{o->threads, -1})) {
@@ -274,6 +280,22 @@ int ref3d_model(void *h, int which, double *out) {
     }
     default: return 0;
     }
+}
+// Detail::memoryBudget (src/two_d/MemoryBudget2D.hpp:12-38): out[0] host bytes, out[1] device bytes
+void ref2d_memory_budget(const orc_params2d *o, int cuda, double *out) {
+    const auto b = S2::Detail::memoryBudget(to2d(*o), cuda ? S3::BackendKind3D::Cuda : S3::BackendKind3D::Cpu);
+    out[0] = b.hostBytes, out[1] = b.deviceBytes;
+}
+// the session's receiver line after initialize (automatic surface layout applied): start, count, stride, depth, target
+void ref2d_receiver_line(void *h, int *out) {
+    const auto &p = static_cast<S2::Simulation *>(h)->parameters();
+    out[0] = p.receiverStart, out[1] = p.receiverCount, out[2] = p.receiverStride, out[3] = p.receiverDepth;
+    out[4] = p.receiverTargetCount;
+}
+// estimateModelMemory3D (include/Seis3D/ModelPresets3D.hpp:38-49): points, cpu, device, cuda host MiB, within budget
+void ref3d_memory_estimate(const orc_params3d *o, double *out) {
+    const auto m = S3::estimateModelMemory3D(to3d(o));
+    out[0] = (double)m.points, out[1] = m.cpuMiB, out[2] = m.deviceMiB, out[3] = m.cudaHostMiB, out[4] = m.withinBudget;
 }
 } // extern "C"
 

@@ -7,7 +7,7 @@ from .params import (Absorber, AbsorbingBoundarySettings3D, BackendKind, CfsPmlS
                      LayerDepthMode3D, MaterialLayer3D, Parameters, Precision, ReceiverArrayKind3D,
                      ReceiverArraySettings3D, SimulationParameters, SourceKind, SourceMechanism, SurfaceGrid3D)
 from .simulation import (AsyncSimulation, Batch2D, Receiver, ReceiverTrace3D, Simulation, Simulation3D,
-                         writeReceivers)
+                         estimateModelMemory3D, memoryBudget, writeReceivers)
 from . import modelio, native, records, workloads
 from .records import ImportedRecords, SegyExportProfile, TraceRecord, read_segy, write_segy
 from .modelio import (ComputeSettings, ModelDocument, SnapshotExportOptions, VtsExportOptions3D,

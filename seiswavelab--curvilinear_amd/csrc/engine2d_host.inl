@@ -523,7 +523,7 @@ template <class R> struct Engine2D {
         for (const auto &x : state) b += x.bytes;
         for (const auto &x : pstate) b += x.bytes;
         for (const DeviceBuffer *x : {&accbuf, &paccbuf, &partials, &level2, &probes, &status, &accepted, &peak, &diag,
-                                      &samples, &gather, &display, &borderNodes})
+                                      &samples, &gather, &display, &borderNodes, &rowTables})
             b += x->bytes;
         return b;
     }

@@ -13,6 +13,10 @@
 #include "model2d.hpp"
 
 #include <cfloat>
+#include <iomanip>
+#include <locale>
+#include <sstream>
+#include <unistd.h>
 #include <vector>
 
 namespace swl {
@@ -59,6 +63,28 @@ __global__ __launch_bounds__(GB) void geometry2d_kernel(Geo2 g, unsigned long lo
 }
 
 } // namespace
+
+// CurvilinearSolver2D.cpp:247-266: the session's budget against 80 % of the free host memory and 85 % of the free
+// memory of the device, with the reference's message.
+void preflightMemory2D(const swl_params2d &p, int device) {
+    const MemoryBudget2D budget = memoryBudget2D(p, 2);
+    const auto require_memory = [](double bytes, double free, double fraction, const char *name) {
+        if (bytes < free * fraction) return;
+        constexpr double gib = 1024. * 1024 * 1024;
+        std::ostringstream message;
+        message.imbue(std::locale::classic());
+        message << "Insufficient " << name << " for a transactional 2D rebuild. " << std::fixed << std::setprecision(2)
+                << "Additional budget: " << bytes / gib << " GiB; available: " << free / gib
+                << " GiB; required free with reserve: " << bytes / fraction / gib << " GiB.";
+        throw std::runtime_error(message.str());
+    };
+    const long pages = sysconf(_SC_AVPHYS_PAGES), page = sysconf(_SC_PAGESIZE);
+    if (pages > 0 && page > 0) require_memory(budget.host, (double)pages * (double)page, .8, "RAM");
+    DeviceScope scope(device);
+    size_t freeBytes = 0, total = 0;
+    check(hipMemGetInfo(&freeBytes, &total), "memory query");
+    require_memory(budget.device, (double)freeBytes, .85, "VRAM");
+}
 
 // Fills m.coordinates, m.a, m.b, m.jac and m.minJacobian, m.maxMetric on `device` from the profiles already in
 // m (surfaceDepth, surfaceSlope, fade, fadeDerivative).  Returns the linear index of the first node with a

@@ -12,6 +12,7 @@
 #include "model3d.hpp"
 
 #include <cfloat>
+#include <unistd.h>
 #include <vector>
 
 namespace swl {
@@ -194,6 +195,23 @@ void buildGeometry3DOnDevice(Model3D &m, const std::vector<double> &surface, con
     m.minJacobian = r.minJ, m.maxMetricNorm = r.maxNorm;
     m.minLength = m.terrain ? r.minLength : m.p.spacing / std::sqrt(3.0);
     m.maxEdge = m.terrain ? r.maxEdge : m.p.spacing;
+}
+
+// fitsMemory3D (include/Seis3D/ModelPresets3D.hpp:50-56) for a HIP session: the host estimate within 80 % of the
+// free host memory and this engine's device bytes within 90 % of the device's free memory, else the reference's
+// refusal (CurvilinearSolver.cpp:309).
+void preflightMemory3D(const swl_params3d &p, int precision, int device) {
+    const ModelMemory3D m = estimateModelMemory3D(p);
+    const long pages = sysconf(_SC_AVPHYS_PAGES), page = sysconf(_SC_PAGESIZE);
+    const double host = pages > 0 && page > 0 ? (double)pages * (double)page : 0;
+    DeviceScope scope(device);
+    size_t freeBytes = 0, total = 0;
+    check(hipMemGetInfo(&freeBytes, &total), "memory query");
+    const bool fits = m.withinBudget && (host == 0 || m.cudaHostMiB * 1024 * 1024 <= .8 * host) &&
+                      hipDeviceBytes3D(p, precision) <= .9 * (double)freeBytes;
+    if (!fits)
+        throw std::runtime_error("Insufficient available memory for the requested grid. Current session retained; reduce "
+                                 "dimensions or use Fit to available memory.");
 }
 
 } // namespace swl

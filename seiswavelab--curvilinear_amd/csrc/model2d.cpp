@@ -199,16 +199,110 @@ void validateGsls(int mechanisms, double fmin, double fmax, double fref, double 
         need(std::isfinite(q) && q >= 5 && q <= 1e9, "Qp and Qs must be finite and in 5..1e9; disable attenuation for the elastic limit.");
 }
 
+namespace {
+// domainLayout (CfsPml2D.hpp:10-14): expanded nx, nz, offsetX, widthX, widthZ
+std::array<int, 5> domainLayout2D(const swl_params2d &p) {
+    const int wx = std::clamp(p.sponge_x, 1, std::max(1, p.nx / 3)), wz = std::clamp(p.sponge_z, 1, std::max(1, p.nz / 3));
+    if (p.absorbing && p.external_absorption) return {p.nx + 2 * wx, p.nz + wz, wx, wx, wz};
+    return {p.nx, p.nz, 0, wx, wz};
+}
+// cpmlScalars (CfsPml2D.hpp:25-29)
+double cpmlScalars2D(const swl_params2d &p) {
+    if (!(p.absorbing && p.absorber == 1)) return 0;
+    const auto d = domainLayout2D(p);
+    return 4.0 * (2.0 * d[3] * d[1] + static_cast<double>(d[4]) * d[0]);
+}
+// setUniformSurfaceReceivers (SurfaceSurvey2D.hpp:8-25), surface depth 0
+bool uniformSurfaceReceivers(swl_params2d &p, int limit, int sideMargin) {
+    if (p.nx < 2 || p.nx > 16384 || p.nz < 2 || limit < 1) return false;
+    const int margin = std::clamp(sideMargin, 0, (p.nx - 2) / 2), span = p.nx - 1 - 2 * margin;
+    const int most = std::min(limit, span + 1), wanted = std::clamp(p.receiver_count, 1, most);
+    int count = 1, stride = 1, start = margin + span / 2;
+    if (wanted > 1) {
+        const int minimumStride = std::max(1, (span + most - 2) / (most - 1));
+        stride = std::max(minimumStride, std::max(1, static_cast<int>(std::lround(static_cast<double>(span) / (wanted - 1)))));
+        count = span / stride + 1;
+        start = margin + (span - (count - 1) * stride) / 2;
+    }
+    p.receiver_start = start, p.receiver_count = count, p.receiver_stride = stride, p.receiver_depth = 0;
+    return true;
+}
+} // namespace
+
+bool synchronizeSurfaceReceivers(swl_params2d &p, int limit, int margin) {
+    if (!p.auto_surface_receivers) return true;
+    if (p.nx < 2 || p.nx > 16384 || p.nz < 2 || p.nz > 16384 || p.receiver_target_count < 0 || p.receiver_target_count > limit)
+        return false;
+    const int target = p.receiver_target_count > 0 ? p.receiver_target_count : p.receiver_count;
+    if (target < 1 || target > limit) return false;
+    const int old = p.receiver_count;
+    p.receiver_count = target;
+    if (!uniformSurfaceReceivers(p, limit, margin)) {
+        p.receiver_count = old;
+        return false;
+    }
+    p.receiver_target_count = target;
+    return true;
+}
+
+MemoryBudget2D memoryBudget2D(const swl_params2d &p, int backend) {
+    const auto d = domainLayout2D(p);
+    const double count = static_cast<double>(d[0] + 6) * (d[1] + 6);
+    const bool device = backend != 0; // the RK state lives on the accelerator (CUDA in the reference, HIP here)
+    const double scalar = p.precision == 1 ? 4.0 : 8.0;
+    const double gradients = p.source_kind == 1 ? 2 : 0;
+    const double samples = p.sample_interval_ms > 0 ? std::floor(p.duration / (p.sample_interval_ms * .001)) + 1
+                                                    : (p.max_steps ? p.max_steps + 1 : 200000 + 1);
+    const double records = p.receiver_count * samples * 2 * sizeof(double);
+    const double qExtra = p.attenuation ? 4.0 + 2.0 * p.mechanisms : 0;
+    const double stateExtra = p.attenuation ? 7.0 * (1 + 4 * p.mechanisms) : 0;
+    const double pmlState = 7. * cpmlScalars2D(p) * scalar;
+    const double pmlProfiles = p.absorbing && p.absorber == 1 ? 3. * (d[0] + d[1]) * scalar : 0;
+    const double hostPerPoint = std::max(((device ? 18. : 60.) + gradients + qExtra + (device ? 0 : stateExtra)) * scalar,
+                                         (14. + gradients + qExtra) * sizeof(double) + 2 * scalar);
+    MemoryBudget2D b;
+    b.host = pmlProfiles + (device ? 0 : pmlState) + count * hostPerPoint + records;
+    if (backend == 1)
+        b.device = pmlState + pmlProfiles + count * (48. + gradients + (p.attenuation ? 2.0 * p.mechanisms : 0) + stateExtra) * scalar;
+    else if (backend == 2)
+        b.device = hipDeviceBytes2D(p);
+    return b;
+}
+
+double hipDeviceBytes2D(const swl_params2d &p, int rows) {
+    // the allocations of Engine2D (engine2d_host.inl) for this session's padded grid
+    const auto d = domainLayout2D(p);
+    const double nx = d[0] + 6, nz = d[1] + 6, nodes = static_cast<double>(d[0]) * d[1];
+    const double word = p.precision == 1 ? 4 : 8;
+    const int mech = p.attenuation ? p.mechanisms : 0, fields = mech ? 6 + 4 * mech : 5;
+    const double ld = std::ceil((nx + 61) / 64) * 64, count = std::ceil((61 + ld * nz) / 64) * 64;
+    const bool sponge = p.absorbing && p.absorber == 0, explosive = p.source_kind == 1;
+    const double cpml = cpmlScalars2D(p), nrec = std::max(p.receiver_count, 1);
+    const double tiles = std::ceil(d[0] / 64.0) * std::ceil(d[1] / 4.0);
+    double bytes = 4.0 * fields * count * word;                                                 // 3 rotating states + accumulator
+    bytes += count * word * (7 + 2 + (sponge ? 1 : 0) + (explosive ? 2 : 0) + 2 * mech);     // mesh arrays (normals: 2)
+    bytes += 8.0 * (6 * nx + 2 * nx + 2 * nz);                                                // FP64 surface row, terrain profiles
+    if (cpml) bytes += 4.0 * cpml * word + 3.0 * (d[0] + d[1]) * word * (word == 4 ? 2 : 1);  // memories, profiles
+    bytes += (3.0 + 2 * mech) * nz * word;                                                     // per-row material tables
+    bytes += 8.0 * nodes;                                                                      // border-node list (bound)
+    bytes += 32.0 * (tiles + std::ceil(nodes / 256) + 1) + 32.0 * 128;                         // inspection partials
+    bytes += 4.0 * nodes;                                                                      // display frame
+    bytes += nrec * (16 + 40) + 16.0 + rows * (4 + 2 * nrec) * 8.0;                            // probes, gather, run rows
+    return bytes;
+}
+
 Model2D buildModel2D(const swl_params2d &requested, int device) {
     Model2D s;
     swl_params2d p = requested;
-    const swl_params2d logical = requested;
     // validation, CurvilinearSolver2D.cpp:161-227
     need(p.precision == 0 || p.precision == 1, "Unknown 2D calculation precision.");
     need(p.nx >= 8 && p.nz >= 8 && p.nx <= 16384 && p.nz <= 16384, "2D dimensions must be in 8..16384 (subject to RAM/VRAM).");
     need(p.receiver_target_count >= 0 && p.receiver_target_count <= 1024, "Desired surface receiver count must be 0..1024.");
-    need(!p.auto_surface_receivers,
-         "Automatic surface receiver layout is not provided by this library; set the receiver line explicitly.");
+    if (p.auto_surface_receivers) { // CurvilinearSolver2D.cpp:164-167: the layout becomes part of the session's parameters
+        const int margin = p.absorbing && !p.external_absorption ? std::clamp(p.sponge_x, 1, std::max(1, p.nx / 3)) : 0;
+        need(synchronizeSurfaceReceivers(p, 1024, margin), "Invalid automatic surface receiver layout.");
+    }
+    const swl_params2d logical = p;
     for (double v : {p.spacing, p.relief, p.wavelength, p.flatten_depth, p.frequency, p.duration, p.cfl,
                      p.force_per_length, p.moment_per_length, p.sponge_strength, p.surface_datum, p.sample_interval_ms})
         need(std::isfinite(v), "2D parameters must be finite.");
@@ -288,6 +382,7 @@ Model2D buildModel2D(const swl_params2d &requested, int device) {
     }
     need(dnx <= 16384 && dnz <= 16384,
          "2D computational dimensions including external absorbing strips must not exceed 16384 per axis.");
+    if (device >= 0) preflightMemory2D(logical, device); // the reference checks here, before anything is built
     p.nx = dnx, p.nz = dnz, p.sponge_x = s.widthX, p.sponge_z = s.widthZ, p.external_absorption = 0;
     sx += s.offsetX;
     p.source_x = sx;

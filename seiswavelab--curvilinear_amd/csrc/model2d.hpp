@@ -70,6 +70,23 @@ Model2D buildModel2D(const swl_params2d &requested, int device = -1);
 long long buildGeometry2DOnDevice(Model2D &m, int device);
 void validateGsls(int mechanisms, double fmin, double fmax, double fref, double qp, double qs);
 swl_mesh2d_desc meshDescriptor(const Model2D &m);
+
+// Detail::memoryBudget (src/two_d/MemoryBudget2D.hpp:12-38): the additional host and device bytes of a session,
+// for the reference's backends (0 CPU, 1 CUDA).  backend 2 (HIP): host bytes as for CUDA (the FP64 model on the
+// host, the RK state on the device) and device bytes bounding what this library's 2D engine allocates.
+struct MemoryBudget2D {
+    double host = 0, device = 0;
+};
+MemoryBudget2D memoryBudget2D(const swl_params2d &p, int backend);
+// Device bytes of a 2D HIP engine for the session p describes, with `rows` per-step result rows reserved (the
+// session steps 512 at a time).  An upper bound: the border-node list is counted at every computational node.
+double hipDeviceBytes2D(const swl_params2d &p, int rows = 512);
+// The reference's admission check before a session is built (CurvilinearSolver2D.cpp:247-266): throws unless the
+// budget is below 80 % of the free host memory and 85 % of the free memory of `device` (geometry2d.hip).
+void preflightMemory2D(const swl_params2d &p, int device);
+// setUniformSurfaceReceivers / synchronizeSurfaceReceivers (include/Seis2D/SurfaceSurvey2D.hpp:8-44) for the
+// elastic session (surface depth 0); false where the reference's returns false.
+bool synchronizeSurfaceReceivers(swl_params2d &p, int limit, int margin);
 double lerpExact(double a, double b, double t); // C++20 std::lerp contract (P0811R3)
 
 } // namespace swl

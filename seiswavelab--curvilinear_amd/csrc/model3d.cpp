@@ -303,7 +303,51 @@ void cflGeometry(Model3D &m) {
 // layer of node q as the material pass assigns it (MaterialModels3D.hpp:19-31)
 int layerOfNode3D(const Model3D &m, size_t q) { return layerAt(m.p, materialDepth(m, q)); }
 
-Model3D buildModel3D(const swl_params3d &requested, int device) {
+ModelMemory3D estimateModelMemory3D(const swl_params3d &p) {
+    ModelMemory3D r;
+    if (p.nx < 8 || p.ny < 8 || p.nz < 8 || p.nx > 4096 || p.ny > 4096 || p.nz > 4096) return r;
+    r.points = static_cast<size_t>(p.nx + 2 * G) * (p.ny + 2 * G) * (p.nz + 2 * G);
+    size_t receivers = 0;
+    try {
+        receivers = receiverNodes(p).size();
+    } catch (const std::exception &) {
+        return r;
+    }
+    const double mib = 1024 * 1024, moment = p.source_kind == 1 ? 3 : 0;
+    const double receiverBytes = receivers * 3.0 * (p.max_steps > 0 ? p.max_steps + 1.0 : 20001.0) * sizeof(double);
+    const double stats = (r.points + 127) / 128 * 72.0;
+    r.cpuMiB = (r.points * ((p.layered ? 84.0 : 81.0) + moment) * sizeof(double) + receiverBytes) / mib;
+    r.cudaHostMiB = (r.points * ((p.layered ? 21.0 : 18.0) + moment) * sizeof(double) + receiverBytes + stats) / mib;
+    r.deviceMiB = (r.points * ((p.layered ? 81.0 : 78.0) + moment) * sizeof(double) + stats) / mib;
+    const double demCopies = 3.0 * (hasDem(p) ? static_cast<double>(p.dem_nx) * p.dem_ny : 0.0) * sizeof(double) / mib;
+    r.cpuMiB += demCopies, r.cudaHostMiB += demCopies;
+    r.withinBudget = r.points <= static_cast<size_t>(306) * 306 * 306;
+    return r;
+}
+
+double hipDeviceBytes3D(const swl_params3d &p, int precision, int rows) {
+    // the allocations of Engine3D (engine3d_host.inl) for this session's padded grid
+    const double nx = p.nx + 2 * G, ny = p.ny + 2 * G, nz = p.nz + 2 * G, N = nx * ny * nz;
+    const double word = precision == 1 ? 4 : 8, tx = 128 / word, ty = 256 / tx;
+    const double ld = std::ceil(nx / 64) * 64, count = std::ceil((61 + ld * ny * nz) / 64) * 64;
+    double receivers = 0;
+    try {
+        receivers = static_cast<double>(receiverNodes(p).size());
+    } catch (const std::exception &) {
+    }
+    const double nrec = std::max(receivers, 1.0);
+    const double tiles = std::ceil(p.nx / tx) * std::ceil(p.ny / ty) * p.nz;
+    double bytes = 4.0 * 9 * count * word;                        // 3 rotating states + accumulator
+    bytes += count * word * (10 + 3 + 1 + 1 + (p.source_kind == 1 ? 3 : 0)); // metrics + J, materials, damping, source
+    bytes += ld * ny * (9 + 1 + 1 + 3) * 8.0;                      // FP64 surface plane
+    bytes += 2 * 88.0 * tiles + 128 * 88.0;                        // inspection partials
+    bytes += 8.0 * N;                                              // field download buffer
+    bytes += 4.0 * std::max({p.nx * p.nz, p.ny * p.nz, p.nx * p.ny}); // display slice
+    bytes += nrec * 3 * 16 + 32.0 + 80.0 + rows * (10 + 3 * nrec) * 8.0; // probes, gather, run rows
+    return bytes;
+}
+
+Model3D buildModel3D(const swl_params3d &requested, int device, int precision) {
     validate(requested);
     Model3D m;
     m.p = requested;
@@ -321,6 +365,7 @@ Model3D buildModel3D(const swl_params3d &requested, int device) {
     m.transition = p.terrain_transition_depth;
     m.terrain = hasDem(p) || p.topography_amplitude != 0;
     validateGeometry(m);
+    if (device >= 0) preflightMemory3D(p, precision, device);
     std::vector<double> surface, fade;
     geometryInputs(m, surface, fade);
     if (device >= 0) {

@@ -43,11 +43,26 @@ struct Model3D {
 
 // Throws std::runtime_error with the reference's messages for invalid input.  device >= 0: the volume
 // geometry and the CFL reductions are computed on that GPU in FP64 (bit-identical to the host loops).
-Model3D buildModel3D(const swl_params3d &requested, int device = -1);
+Model3D buildModel3D(const swl_params3d &requested, int device = -1, int precision = 0);
 // geometry3d.hip: coordinates, metrics, Jacobian, normals and the CFL figures from the per-column surface
 // displacement and the per-plane fade.
 void buildGeometry3DOnDevice(Model3D &m, const std::vector<double> &surface, const std::vector<double> &fade, int device);
 swl_mesh3d_desc meshDescriptor3D(const Model3D &m, int precision);
 int layerOfNode3D(const Model3D &m, size_t q); // layered models: the node's layer index
+
+// estimateModelMemory3D (include/Seis3D/ModelPresets3D.hpp:38-49): padded points, CPU MiB, CUDA device MiB,
+// CUDA host MiB, and whether the grid is within the padded-node budget.
+struct ModelMemory3D {
+    size_t points = 0;
+    double cpuMiB = 0, deviceMiB = 0, cudaHostMiB = 0;
+    bool withinBudget = false;
+};
+ModelMemory3D estimateModelMemory3D(const swl_params3d &p);
+// Device bytes of a 3D HIP engine for the session p describes, with `rows` per-step result rows reserved (the
+// session steps 256 at a time).  An upper bound: metrics, materials and damping are counted on every node.
+double hipDeviceBytes3D(const swl_params3d &p, int precision, int rows = 256);
+// fitsMemory3D (ModelPresets3D.hpp:50-56) for a HIP session on `device`, with the reference's message
+// (CurvilinearSolver.cpp:309); implemented in geometry3d.hip.
+void preflightMemory3D(const swl_params3d &p, int precision, int device);
 
 } // namespace swl

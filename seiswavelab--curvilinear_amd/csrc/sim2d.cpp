@@ -91,6 +91,12 @@ int swl_sim2d_reset(swl_sim2d *s) {
     });
 }
 int swl_sim2d_step(const swl_sim2d *s) { return s->step; }
+int swl_sim2d_parameters(const swl_sim2d *s, swl_params2d *out) {
+    *out = s->model.logical;
+    out->node_vp = out->node_vs = out->node_density = nullptr;                             // not echoed
+    out->surface_xz = s->model.surfaceSamples.empty() ? nullptr : s->model.surfaceSamples.data(); // the session's copy
+    return 0;
+}
 int swl_sim2d_override_peak_energy(swl_sim2d *s, double peak) {
     return swl::guarded([&] {
         swl::require(std::isfinite(peak) && peak >= 0, "peak energy must be finite and non-negative");
@@ -239,6 +245,13 @@ int64_t swl_pml_slot2d(int nx, int nz, int wx, int wz, int64_t q, int axis) {
         return j < 0 ? -1 : 4 * (static_cast<int64_t>(k) * 2 * wx + j);
     }
     return k < nzc - wz ? -1 : 4 * (static_cast<int64_t>(2 * wx) * nzc + static_cast<int64_t>(k - (nzc - wz)) * nxc + i);
+}
+int swl_memory_budget2d(const swl_params2d *p, int backend, double *out) {
+    return swl::guarded([&] {
+        swl::require(p && out && backend >= 0 && backend <= 2, "invalid memory budget arguments");
+        const auto b = swl::memoryBudget2D(*p, backend);
+        out[0] = b.host, out[1] = b.device;
+    });
 }
 size_t swl_params2d_size(void) { return sizeof(swl_params2d); }
 size_t swl_params3d_size(void) { return sizeof(swl_params3d); }

@@ -20,7 +20,7 @@ int swl_sim3d_create(const swl_params3d *p, int backend, int device, int precisi
         swl::require(precision == 0 || precision == 1, "3D precision must be 0 (FP64) or 1 (FP32).");
         const auto begin = std::chrono::steady_clock::now();
         auto s = std::make_unique<swl_sim3d>();
-        s->model = swl::buildModel3D(*p, device); // FP64 volume geometry and CFL on the device
+        s->model = swl::buildModel3D(*p, device, precision); // FP64 volume geometry and CFL on the device
         s->device = device, s->precision = precision;
         const auto desc = swl::meshDescriptor3D(s->model, precision);
         if (swl_engine3d_create(&desc, s->model.receiverIndex.data(), static_cast<int>(s->model.receiverIndex.size()), device,
@@ -29,6 +29,14 @@ int swl_sim3d_create(const swl_params3d *p, int backend, int device, int precisi
         s->clearRecords();
         s->initializationMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - begin).count();
         *out = s.release();
+    });
+}
+int swl_memory_estimate3d(const swl_params3d *p, int precision, double *out) {
+    return swl::guarded([&] {
+        swl::require(p && out && (precision == 0 || precision == 1), "invalid memory estimate arguments");
+        const auto m = swl::estimateModelMemory3D(*p);
+        out[0] = (double)m.points, out[1] = m.cpuMiB, out[2] = m.deviceMiB, out[3] = m.cudaHostMiB;
+        out[4] = m.withinBudget ? 1 : 0, out[5] = m.points ? swl::hipDeviceBytes3D(*p, precision) : 0;
     });
 }
 int swl_sim3d_plan(const swl_params3d *p, swl_sim3d **out) {

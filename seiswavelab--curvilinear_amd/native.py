@@ -62,6 +62,9 @@ SIGNATURES = {
     "swl_sim2d_reset": (C.c_int, [vp]),
     "swl_sim2d_step": (C.c_int, [vp]),
     "swl_sim2d_override_peak_energy": (C.c_int, [vp, C.c_double]),
+    "swl_memory_budget2d": (C.c_int, [vp, C.c_int, vp]),
+    "swl_sim2d_parameters": (C.c_int, [vp, vp]),
+    "swl_memory_estimate3d": (C.c_int, [vp, C.c_int, vp]),
     "swl_sim2d_total_steps": (C.c_int, [vp]),
     "swl_sim2d_dt": (C.c_double, [vp]),
     "swl_sim2d_record_dt": (C.c_double, [vp]),

@@ -22,25 +22,13 @@ from typing import Dict, List, Optional
 import numpy as np
 
 from .params import Absorber, BackendKind, Parameters, Precision
-from .simulation import Batch2D, Simulation
+from .simulation import Batch2D, Simulation, memoryBudget
 
 
 def footprint_bytes(p: Parameters) -> int:
-    """Device bytes of a session: three rotating state buffers + accumulator (GSLS: 6 + 4L fields),
-    the mesh arrays, PML memories and row padding (the engine's aligned layout)."""
-    w = min(max(p.spongeX, 1), max(1, p.nx // 3)) if p.absorbing else 0
-    wz = min(max(p.spongeZ, 1), max(1, p.nz // 3)) if p.absorbing else 0
-    nx, nz = p.nx, p.nz
-    if p.externalAbsorption and p.absorbing:
-        nx, nz = nx + 2 * w, nz + wz
-    ld = (nx + 6 + 61 + 63) // 64 * 64
-    count = ld * (nz + 6)
-    mech = p.attenuation.mechanisms if p.attenuation.enabled else 0
-    fields = 6 + 4 * mech if mech else 5
-    word = 4 if p.precision == Precision.Float32 else 8
-    mesh_arrays = 11 + 2 * mech
-    pml = 4 * (2 * w * nz + wz * nx) * 4 * word if p.absorber == Absorber.CfsPml else 0
-    return int(count * word * (4 * fields + mesh_arrays) + pml + (16 << 20))
+    """Device bytes a session of p allocates at most: the HIP engine's budget, the device part of the reference's
+    per-session memory model (MemoryBudget2D.hpp:12-38) computed for this library's engine."""
+    return int(memoryBudget(p, BackendKind.Hip)["deviceBytes"])
 
 
 @dataclass

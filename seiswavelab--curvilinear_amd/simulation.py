@@ -7,6 +7,7 @@ All stepping runs in the HIP library; this class only marshals arguments.
 """
 from __future__ import annotations
 
+import copy
 import ctypes as C
 from dataclasses import dataclass
 from typing import List, Tuple
@@ -14,7 +15,7 @@ from typing import List, Tuple
 import numpy as np
 
 from . import native
-from .params import BackendKind, Parameters, to_c
+from .params import BackendKind, CParams2D, Parameters, to_c
 
 DIAGNOSTIC_NAMES = ["energy", "peakEnergy", "maxVelocity", "tractionRelative", "finite", "waveDt", "dampingDt",
                     "sourceDt", "cpmlDt", "relaxationDt", "ppw", "maxRelativeQError", "maxMetric", "minJacobian",
@@ -62,6 +63,12 @@ class Simulation:
         self.close()
         self._h = out
         self._params = params
+        if params.autoSurfaceReceivers:  # the session's parameters carry the layout it chose
+            e = CParams2D()
+            lib.swl_sim2d_parameters(out, C.byref(e))
+            q = self._params = copy.deepcopy(params)
+            q.receiverStart, q.receiverCount, q.receiverStride = e.receiver_start, e.receiver_count, e.receiver_stride
+            q.receiverDepth, q.receiverTargetCount = e.receiver_depth, e.receiver_target_count
         return True, ""
 
     def close(self):
@@ -372,6 +379,27 @@ class AsyncSimulation:
 
     def snapshot(self, includeMemories: bool = False) -> dict:
         return self.session().snapshot(includeMemories)
+
+
+def memoryBudget(params: Parameters, backend: BackendKind = BackendKind.Hip) -> dict:
+    """Detail::memoryBudget (src/two_d/MemoryBudget2D.hpp:12-38): the host and device bytes a 2D session adds, for
+    the reference's CPU and CUDA backends as it computes them, and for this library's HIP engine (device bytes an
+    upper bound of the engine's allocations)."""
+    c, keep = to_c(params)
+    o = np.zeros(2)
+    native.call("swl_memory_budget2d", C.byref(c), int(backend), _dp(o))
+    return {"hostBytes": float(o[0]), "deviceBytes": float(o[1])}
+
+
+def estimateModelMemory3D(params, precision: int = 0) -> dict:
+    """estimateModelMemory3D (include/Seis3D/ModelPresets3D.hpp:38-49), plus the HIP engine's device bytes at
+    `precision` (0 FP64, 1 FP32)."""
+    from .params import to_c3d
+    c, keep = to_c3d(params)
+    o = np.zeros(6)
+    native.call("swl_memory_estimate3d", C.byref(c), int(precision), _dp(o))
+    return {"points": int(o[0]), "cpuMiB": float(o[1]), "deviceMiB": float(o[2]), "cudaHostMiB": float(o[3]),
+            "withinBudget": bool(o[4]), "hipDeviceBytes": float(o[5])}
 
 
 def writeReceivers(sim: Simulation, path: str):
