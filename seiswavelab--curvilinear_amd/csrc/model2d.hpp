@@ -79,7 +79,7 @@ struct MemoryBudget2D {
 };
 MemoryBudget2D memoryBudget2D(const swl_params2d &p, int backend);
 // Device bytes of a 2D HIP engine for the session p describes, with `rows` per-step result rows reserved (the
-// session steps 512 at a time).  An upper bound: the border-node list is counted at every computational node.
+// session steps 512 at a time; fewer when max_steps is smaller).  An upper bound: the border-node list is counted at every computational node.
 double hipDeviceBytes2D(const swl_params2d &p, int rows = 512);
 // The reference's admission check before a session is built (CurvilinearSolver2D.cpp:247-266): throws unless the
 // budget is below 80 % of the free host memory and 85 % of the free memory of `device` (geometry2d.hip).

@@ -337,6 +337,7 @@ double hipDeviceBytes3D(const swl_params3d &p, int precision, int rows) {
     }
     const double nrec = std::max(receivers, 1.0);
     const double tiles = std::ceil(p.nx / tx) * std::ceil(p.ny / ty) * p.nz;
+    if (p.max_steps > 0) rows = std::min(rows, p.max_steps); // a run never reserves more rows than it has steps
     double bytes = 4.0 * 9 * count * word;                        // 3 rotating states + accumulator
     bytes += count * word * (10 + 3 + 1 + 1 + (p.source_kind == 1 ? 3 : 0)); // metrics + J, materials, damping, source
     bytes += ld * ny * (9 + 1 + 1 + 3) * 8.0;                      // FP64 surface plane

@@ -59,7 +59,7 @@ struct ModelMemory3D {
 };
 ModelMemory3D estimateModelMemory3D(const swl_params3d &p);
 // Device bytes of a 3D HIP engine for the session p describes, with `rows` per-step result rows reserved (the
-// session steps 256 at a time).  An upper bound: metrics, materials and damping are counted on every node.
+// session steps 256 at a time; fewer when max_steps is smaller).  An upper bound: metrics, materials and damping are counted on every node.
 double hipDeviceBytes3D(const swl_params3d &p, int precision, int rows = 256);
 // fitsMemory3D (ModelPresets3D.hpp:50-56) for a HIP session on `device`, with the reference's message
 // (CurvilinearSolver.cpp:309); implemented in geometry3d.hip.
