@@ -501,6 +501,34 @@ int swl_sim3d_write_receivers_vz_csv(const swl_sim3d *s, const char *path, int a
 int swl_sim3d_write_vts(const swl_sim3d *s, const char *path, const int32_t *fields, int include_material,
                         int include_velocity_vector);
 
+/* ---------------- 3D background stepping (Seis3D::Curvilinear::AsyncSimulation3D) ----------------
+   include/Seis3D/AsyncSimulation3D.hpp:5-33, src/three_d/AsyncSimulation3D.cpp: queue(maxSteps, budgetMs) runs at
+   most 8 steps on a worker thread; poll / synchronize publish its step, diagnostics, the Vx, Vy, Vz samples it
+   recorded and the requested display slices; every other call first cancels the batch between steps and waits. */
+typedef struct swl_async3d swl_async3d;
+int swl_async3d_create(swl_async3d **out);
+void swl_async3d_destroy(swl_async3d *a);
+int swl_async3d_initialize(swl_async3d *a, const swl_params3d *p, int backend, int device, int precision);
+int swl_async3d_queue(swl_async3d *a, int steps, double budget_ms); /* 1 queued, 0 busy / completed / uninitialized */
+int swl_async3d_poll(swl_async3d *a);          /* non-zero: the last batch's error (swl_last_error) */
+void swl_async3d_pause(swl_async3d *a);
+int swl_async3d_synchronize(swl_async3d *a);
+int swl_async3d_busy(swl_async3d *a);
+/* backgroundTiming: compute ms, transfer ms, batches, steps */
+int swl_async3d_timing(const swl_async3d *a, double *out);
+int swl_async3d_advance(swl_async3d *a, int count);
+int swl_async3d_reset(swl_async3d *a); /* resetFields */
+/* published state: initialized, completed, step, total, receivers, samples per receiver */
+int swl_async3d_state(const swl_async3d *a, int64_t *out);
+double swl_async3d_dt(const swl_async3d *a);
+int swl_async3d_diagnostics(const swl_async3d *a, double *out); /* as swl_sim3d_diagnostics */
+int swl_async3d_receivers(const swl_async3d *a, double *vx, double *vy, double *vz);
+/* readDisplayFrame (AsyncSimulation3D.cpp:86-92): the published slices of the current step, else (worker idle)
+   read now; *ready = 0 while a batch runs and no current slices exist (the request is kept for the next batch). */
+int swl_async3d_read_display_frame(swl_async3d *a, int x, int y, int z, int component, float *out, int *ready);
+/* the session (owned by the async object), after the running batch finishes */
+swl_sim3d *swl_async3d_session(swl_async3d *a);
+
 #ifdef __cplusplus
 }
 #endif

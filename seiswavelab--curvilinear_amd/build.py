@@ -20,7 +20,7 @@ LIB = CSRC / "libswl.so"
 
 HIP_SOURCES = ["engine2d.hip", "engine3d.hip", "geometry2d.hip", "geometry3d.hip"]
 GCC_OPENMP_SOURCES = {"vts2d.cpp", "vts3d.cpp"}
-CXX_SOURCES = ["model2d.cpp", "sim2d.cpp", "model3d.cpp", "sim3d.cpp", "batch2d.cpp", "segy.cpp", "async2d.cpp", "vts2d.cpp", "vts3d.cpp"]
+CXX_SOURCES = ["model2d.cpp", "sim2d.cpp", "model3d.cpp", "sim3d.cpp", "batch2d.cpp", "segy.cpp", "async2d.cpp", "async3d.cpp", "vts2d.cpp", "vts3d.cpp"]
 
 
 def hipcc() -> str:

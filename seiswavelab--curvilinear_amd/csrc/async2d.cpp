@@ -8,6 +8,7 @@
 // a finished publication into the caller's copy, so readers on the caller's thread only ever see
 // published state.  synchronize() (and every call that touches the session itself) cancels the batch
 // between steps and waits for it, as SeisWave::BackgroundRunner::wait does.
+#include "background.hpp"
 #include "session2d.hpp"
 
 #include <atomic>
@@ -35,86 +36,11 @@ struct Publication {
     std::string error;
 };
 
-// One persistent worker; only the owner consumes finished results; cancellation between steps.
-class Worker {
-    std::mutex mutex_;
-    std::condition_variable wake_, done_;
-    std::function<Publication(const std::atomic_bool &)> task_;
-    std::optional<Publication> result_;
-    std::string failure_;
-    std::atomic_bool cancel_{false};
-    bool busy_ = false, ready_ = false, shutdown_ = false;
-    std::thread thread_;
-
-    std::optional<Publication> consume() { // mutex held
-        if (!ready_) return {};
-        ready_ = busy_ = false;
-        if (!failure_.empty()) throw std::runtime_error(std::exchange(failure_, {}));
-        auto r = std::move(result_);
-        result_.reset();
-        return r;
-    }
-
-public:
-    Worker()
-        : thread_([this] {
-              for (;;) {
-                  std::unique_lock<std::mutex> lock(mutex_);
-                  wake_.wait(lock, [&] { return shutdown_ || static_cast<bool>(task_); });
-                  if (shutdown_) return;
-                  auto task = std::move(task_);
-                  task_ = nullptr;
-                  lock.unlock();
-                  std::optional<Publication> value;
-                  std::string failure;
-                  try {
-                      value = task(cancel_);
-                  } catch (const std::exception &e) {
-                      failure = e.what();
-                  }
-                  lock.lock();
-                  result_ = std::move(value), failure_ = failure, ready_ = true;
-                  done_.notify_all();
-              }
-          }) {}
-    ~Worker() {
-        cancel();
-        {
-            std::unique_lock<std::mutex> lock(mutex_);
-            done_.wait(lock, [&] { return !busy_ || ready_; });
-            shutdown_ = true;
-            wake_.notify_all();
-        }
-        thread_.join();
-    }
-    bool busy() {
-        std::lock_guard<std::mutex> lock(mutex_);
-        return busy_;
-    }
-    bool submit(std::function<Publication(const std::atomic_bool &)> task) {
-        std::lock_guard<std::mutex> lock(mutex_);
-        if (busy_) return false;
-        cancel_ = false, busy_ = true, task_ = std::move(task);
-        wake_.notify_one();
-        return true;
-    }
-    void cancel() { cancel_ = true; }
-    std::optional<Publication> poll() {
-        std::lock_guard<std::mutex> lock(mutex_);
-        return consume();
-    }
-    std::optional<Publication> wait() {
-        cancel();
-        std::unique_lock<std::mutex> lock(mutex_);
-        done_.wait(lock, [&] { return !busy_ || ready_; });
-        return consume();
-    }
-};
 } // namespace
 
 struct swl_async2d {
     swl_sim2d *core = nullptr; // owned
-    Worker worker;
+    swl::BackgroundWorker<Publication> worker;
     // the caller's published copy
     bool initialized = false;
     int step = 0, total = 0;

@@ -64,6 +64,23 @@ SIGNATURES = {
     "swl_sim2d_override_peak_energy": (C.c_int, [vp, C.c_double]),
     "swl_memory_budget2d": (C.c_int, [vp, C.c_int, vp]),
     "swl_sim2d_parameters": (C.c_int, [vp, vp]),
+    "swl_async3d_create": (C.c_int, [P(vp)]),
+    "swl_async3d_destroy": (None, [vp]),
+    "swl_async3d_initialize": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int]),
+    "swl_async3d_queue": (C.c_int, [vp, C.c_int, C.c_double]),
+    "swl_async3d_poll": (C.c_int, [vp]),
+    "swl_async3d_pause": (None, [vp]),
+    "swl_async3d_synchronize": (C.c_int, [vp]),
+    "swl_async3d_busy": (C.c_int, [vp]),
+    "swl_async3d_timing": (C.c_int, [vp, vp]),
+    "swl_async3d_advance": (C.c_int, [vp, C.c_int]),
+    "swl_async3d_reset": (C.c_int, [vp]),
+    "swl_async3d_state": (C.c_int, [vp, vp]),
+    "swl_async3d_dt": (C.c_double, [vp]),
+    "swl_async3d_diagnostics": (C.c_int, [vp, vp]),
+    "swl_async3d_receivers": (C.c_int, [vp, vp, vp, vp]),
+    "swl_async3d_read_display_frame": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp]),
+    "swl_async3d_session": (vp, [vp]),
     "swl_memory_estimate3d": (C.c_int, [vp, C.c_int, vp]),
     "swl_sim2d_total_steps": (C.c_int, [vp]),
     "swl_sim2d_dt": (C.c_double, [vp]),

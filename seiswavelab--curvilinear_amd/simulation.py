@@ -567,6 +567,129 @@ class Simulation3D:
                 "metric_planes": int(o[4]), "material_by_depth": bool(o[5])}
 
 
+class AsyncSimulation3D:
+    """Background stepping of a 3D session: the reference's AsyncSimulation3D (AsyncSimulation3D.hpp:5-33) over
+    swl_async3d_*.  ``queue(maxSteps, budgetMs)`` runs at most 8 steps on a worker thread; ``poll`` /
+    ``synchronize`` fold the published batch (step, diagnostics, Vx / Vy / Vz samples, requested display slices)
+    into the state the accessors return; ``pause`` stops a batch between steps."""
+
+    def __init__(self, backend: BackendKind = BackendKind.Hip):
+        out = C.c_void_p()
+        native.call("swl_async3d_create", C.byref(out))
+        self._h = out
+        self._params = None
+        self._backend = backend
+
+    def close(self):
+        if self._h:
+            native.lib().swl_async3d_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _state(self):
+        o = np.zeros(6, np.int64)
+        native.lib().swl_async3d_state(self._h, _dp(o))
+        return o
+
+    def initialize(self, params, device: int = 0, precision: int = 0):
+        from .params import to_c3d
+        c, keep = to_c3d(params)
+        rc = native.lib().swl_async3d_initialize(self._h, C.byref(c), int(self._backend), int(device), int(precision))
+        if rc != 0:
+            return False, native.last_error()
+        self._params = params
+        return True, ""
+
+    def queue(self, maxSteps: int, budgetMs: float) -> bool:
+        return native.lib().swl_async3d_queue(self._h, int(maxSteps), float(budgetMs)) == 1
+
+    def poll(self):
+        rc = native.lib().swl_async3d_poll(self._h)
+        return (rc == 0), ("" if rc == 0 else native.last_error())
+
+    def pause(self):
+        native.lib().swl_async3d_pause(self._h)
+
+    def synchronize(self):
+        if native.lib().swl_async3d_synchronize(self._h) != 0:
+            raise RuntimeError(native.last_error())
+
+    def busy(self) -> bool:
+        return native.lib().swl_async3d_busy(self._h) == 1
+
+    def backgroundTiming(self) -> dict:
+        o = np.zeros(4)
+        native.lib().swl_async3d_timing(self._h, _dp(o))
+        return {"computeMs": o[0], "transferMs": o[1], "batches": int(o[2]), "steps": int(o[3])}
+
+    def advance(self, count: int):
+        rc = native.lib().swl_async3d_advance(self._h, int(count))
+        return (rc == 0), ("" if rc == 0 else native.last_error())
+
+    def resetFields(self):
+        rc = native.lib().swl_async3d_reset(self._h)
+        return (rc == 0), ("" if rc == 0 else native.last_error())
+
+    def initialized(self) -> bool:
+        return bool(self._state()[0])
+
+    def completed(self) -> bool:
+        return bool(self._state()[1])
+
+    def currentStep(self) -> int:
+        return int(self._state()[2])
+
+    def totalSteps(self) -> int:
+        return int(self._state()[3])
+
+    def dt(self) -> float:
+        return float(native.lib().swl_async3d_dt(self._h))
+
+    def parameters(self):
+        return self._params
+
+    def diagnostics(self) -> dict:
+        o = np.zeros(len(DIAGNOSTIC_NAMES_3D))
+        native.lib().swl_async3d_diagnostics(self._h, _dp(o))
+        d = dict(zip(DIAGNOSTIC_NAMES_3D, o.tolist()))
+        d["allFinite"] = bool(d["allFinite"])
+        return d
+
+    def receiver_arrays(self):
+        s = self._state()
+        nr, n = int(s[4]), int(s[5])
+        vx, vy, vz = np.zeros((nr, n)), np.zeros((nr, n)), np.zeros((nr, n))
+        native.lib().swl_async3d_receivers(self._h, _dp(vx), _dp(vy), _dp(vz))
+        return {"vx": vx, "vy": vy, "vz": vz}
+
+    def readDisplayFrame(self, x: int, y: int, z: int, component: int):
+        """(ok, error, planes).  While a batch runs and the current step's slices are not yet published it returns
+        (False, "", None) and keeps the request for the next batch, as the reference does."""
+        p = self._params
+        out = np.zeros(p.nx * p.nz + p.ny * p.nz + p.nx * p.ny, np.float32)
+        ready = C.c_int()
+        rc = native.lib().swl_async3d_read_display_frame(self._h, int(x), int(y), int(z), int(component), _dp(out),
+                                                          C.byref(ready))
+        if rc != 0:
+            return False, native.last_error(), None
+        if not ready.value:
+            return False, "", None
+        a, b = p.nx * p.nz, p.ny * p.nz
+        return True, "", [out[:a].reshape(p.nz, p.nx), out[a:a + b].reshape(p.nz, p.ny), out[a + b:].reshape(p.ny, p.nx)]
+
+    def field(self, component: int) -> np.ndarray:
+        """One field of the accepted state (padded storage), after the running batch finishes (snapshot())."""
+        h = native.lib().swl_async3d_session(self._h)
+        o = np.zeros(int(np.prod([self._params.nx + 6, self._params.ny + 6, self._params.nz + 6])))
+        native.call("swl_sim3d_snapshot", h, int(component), _dp(o))
+        return o
+
+
 class Batch2D:
     """Steps many 2D sessions together (one launch per RK stage for all of them; graph-captured chunks).
 

@@ -82,3 +82,27 @@ def test_engine_rollback_and_reset():
     e.reset()
     assert all(not np.any(e.snapshot(f)) for f in range(5))
     assert e.residentBytes() > 0
+
+
+@pytest.mark.parametrize("name", ["sine_terrain_small", "moment_external_pml"])
+def test_engine_from_fp32_rounded_mesh(name):
+    """The reference's FP32 façade hands CudaEngine a MeshT<float> whose every array is rounded to FP32
+    (CurvilinearSolver2D.cpp:498-513); the adapter widens those arrays back to FP64.  Such an engine with FP32 state
+    stays within the FP32 bar of the FP64 oracle."""
+    p = ref_case(name)
+    p.precision = Precision.Float64
+    o = CpuSession2D("orc", p)
+    mesh = mesh_from_arrays(o.model, o.layout(), p, precision=1)
+    rounded = {k: (np.asarray(v, np.float64).astype(np.float32).astype(np.float64) if isinstance(v, np.ndarray) else v)
+               for k, v in mesh.items()}
+    if mesh.get("gradient") is not None:
+        rounded["gradient"] = tuple(g.astype(np.float32).astype(np.float64) for g in mesh["gradient"])
+    rounded["h"] = float(np.float32(p.spacing))
+    e = Engine2D(rounded, o.probe_indices(), 0)
+    raw, _ = advance(e, p, o.dt, o.total)
+    assert o.advance(o.total)[0]
+    rec = o.receivers()
+    got = np.concatenate([raw[:, :, 0].T.ravel(), raw[:, :, 1].T.ravel()])
+    want = np.concatenate([rec["vx"][:, 1:].ravel(), rec["vz"][:, 1:].ravel()])
+    assert rel_l2(want, got) <= 1e-4
+    e.close()
