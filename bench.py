@@ -197,12 +197,16 @@ def cpu_baseline(pkg, params, steps=4, warmup=1, repeats=3):
     cpath = ROOT / "oracle" / "calibration.json"
     if cpath.exists():
         calib = json.loads(cpath.read_text())
+    ratio = calib["cases"]["elastic_cfs_pml_2048"]["oracle_over_reference"] if calib else None
     return {"value": runs[best]["value"], "unit": "G pt-updates/s", "cores": best, "kind": "port",
+            "reference_equivalent_value": round(runs[best]["value"] * ratio, 5) if ratio else None,
             "sample": f"same 5061x2031-node model in FP64 (the reference's CPU precision), median of {repeats} runs "
                       f"of {steps} production advance() steps (RHS, combine, projection, energy inspection, "
                       f"receivers) each after {warmup} warm-up step(s), OpenMP as the reference; thread counts "
                       f"min(8, nproc) and min(64, physical cores)",
-            "by_threads": runs, "host": info, "oracle_vs_reference_build": calib}
+            "by_threads": runs, "host": info, "oracle_vs_reference_build": calib,
+            "note": "reference_equivalent_value = value x the oracle/reference per-step time ratio measured on the "
+                    "2048^2 elastic CFS-PML case at 8 threads where both builds exist (oracle/calibration.json)"}
 
 
 ALGO_BYTES_3D = {"f32": 784.0, "f64": 1568.0}  # per node per RK4 step, 3D nine-field (9 fields, 13 coefficients)
