@@ -452,6 +452,15 @@ int swl_engine3d_run(swl_engine3d *e, int first_step, int count, double dt, doub
    stage launches, nodes per launch, VGPRs, metric planes stored, material by depth (0/1). */
 int swl_engine3d_profile(swl_engine3d *e, int enable);
 int swl_engine3d_kernel_timing(swl_engine3d *e, double *out);
+/* Engine tuning switches, all with identical fields (the energy sum may differ in its last bits, summation order):
+   "vector_terrain", "vector_flat": x-adjacent nodes per lane of the terrain and flat bands, read and written as
+   one vector (2, or 4 in FP32; 0 or 1: one node per lane).  Defaults: FP64 2 / 2, FP32 2 / 4.
+   "overlap_bands" (default 1): the surface band runs on a second stream beside the deep bands of the same stage.
+   "split_surface" (default 0): one launch per surface plane, each compiled for its depth.
+   "lds_tiles" (default 0): the one-node-per-lane deep bands stage each field's tile and x / y halo in LDS.
+   "brick_x", "brick_y", "brick_z": the deep bands' tile order within each XCD's run, bricks of x by y tiles by
+   z planes (brick_x 0 = tile columns, the default). */
+int swl_engine3d_set_option(swl_engine3d *e, const char *name, int value);
 
 /* ---------------- 3D session (ISimulationBackend3D, include/Seis3D/CurvilinearSolver.hpp:165-185) ---------------- */
 typedef struct swl_sim3d swl_sim3d;
@@ -488,6 +497,7 @@ int swl_sim3d_model(const swl_sim3d *s, int which, double *out);
 int swl_sim3d_read_display_frame(const swl_sim3d *s, int x, int y, int z, int component, float *out);
 int swl_sim3d_profile(swl_sim3d *s, int enable);
 int swl_sim3d_kernel_timing(const swl_sim3d *s, double *out);
+int swl_sim3d_set_option(swl_sim3d *s, const char *name, int value); /* swl_engine3d_set_option */
 /* writeReceiversCsv(simulation, path, components, error) (include/Seis3D/ReceiverIo.hpp:5-7,
    src/three_d/ReceiverIo.cpp:9-51): components[3] = Vx, Vy, Vz; one row per accepted step, t = 0 included. */
 int swl_sim3d_write_receivers_csv(const swl_sim3d *s, const char *path, const int32_t *components);

@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 namespace swl {
@@ -76,6 +77,26 @@ __device__ void block_reduce3(Partial3 &p, Partial3 *sh) {
     }
 }
 
+// Block reduction through wave shuffles and one partial per wave in LDS (4 x 88 bytes instead of 256 x 88).
+__device__ inline double shfl_down_d(double v, int off) { return __shfl_down(v, off, 64); }
+__device__ void block_reduce3_waves(Partial3 &p, Partial3 *sh4) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        Partial3 o;
+        o.energy = shfl_down_d(p.energy, off), o.velocity = shfl_down_d(p.velocity, off);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) o.peak[c] = shfl_down_d(p.peak[c], off);
+        o.traction = shfl_down_d(p.traction, off), o.norm = shfl_down_d(p.norm, off);
+        o.bad = __shfl_down(p.bad, off, 64), o.pad = 0;
+        merge3(p, o);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) sh4[wave] = p;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int w = 1; w < THREADS / 64; ++w) merge3(p, sh4[w]);
+}
+
 template <class R> struct Stage3 {
     DMesh3<R> m;
     const R *in, *base;
@@ -88,7 +109,30 @@ template <class R> struct Stage3 {
     Partial3 *partials;
     int ntx, nty, ntiles; // tiles of TX x TY nodes per plane row / column, tiles of all planes
     int t0, nband;        // this launch covers tiles [t0, t0 + nband) (a band of k planes)
+    int lds;              // deep bands: stage each field's tile and x / y halo in LDS (else direct loads)
+    int bx, by, bz;       // tile bricks per XCD run (bx x by tiles by bz planes); bx = 0: whole tile columns
+    int k0;               // the band's first plane
 };
+
+// Position of the u-th block of an XCD-contiguous run within a band of `planes` planes of ntx x nty tiles.
+// bx = 0: tile columns, planes fastest.  Otherwise bricks of bx x by tiles by bz planes, in patch rows, patches,
+// slabs; planes fastest within a brick, then x tiles, then y tiles.  Returns the tile's plane-major index.
+__device__ inline int tile_in_band(int u, int planes, int ntx, int nty, int bx, int by, int bz) {
+    const int perPlane = ntx * nty;
+    if (bx <= 0) return (u % planes) * perPlane + u / planes;
+    const int rowBlocks = by * ntx * planes, npy = (nty + by - 1) / by;
+    const int py = min(u / rowBlocks, npy - 1), hy = min(by, nty - py * by);
+    int r = u - py * rowBlocks;
+    const int patchBlocks = bx * hy * planes, npx = (ntx + bx - 1) / bx;
+    const int px = min(r / patchBlocks, npx - 1), hx = min(bx, ntx - px * bx);
+    r -= px * patchBlocks;
+    const int slabBlocks = hx * hy * bz, ns = (planes + bz - 1) / bz;
+    const int s = min(r / slabBlocks, ns - 1), hz = min(bz, planes - s * bz);
+    r -= s * slabBlocks;
+    const int plane = s * bz + r % hz, cell = r / hz;
+    const int tx = px * bx + cell % hx, ty = py * by + cell / hx;
+    return plane * perPlane + ty * ntx + tx;
+}
 
 __device__ inline int tile_of_block(int b, int n) { // XCD-contiguous band of tiles
     const int x = b & 7, j = b >> 3, q = n >> 3, r = n & 7;
@@ -327,20 +371,176 @@ __device__ __forceinline__ void deep_node(const Stage3<R> &A, long long q, int i
     }
 }
 
+// The deep-band body with each field's stencil input staged in LDS: the block's tile plus 4 columns on each side and
+// the y halo of the stencil direction (1 and 3 rows), loaded once with contiguous row reads, so the x and y taps
+// come from LDS and only the z taps are global loads (aligned whole rows).  Two buffers: the next field is loaded
+// while the current one is used, one barrier per field.  The arithmetic is deep_node's, term for term.
+template <class R, int DY> struct LdsTile {
+    static constexpr int W = TX<R> + 8, YLO = DY ? 3 : 1, H = TY<R> + 4, N = W * H;
+};
 template <class R, int DX, int DY, int DZ, int BAND>
+__device__ __forceinline__ void deep_node_lds(const Stage3<R> &A, long long q0, int tx, int ty, bool active, int i, int j,
+                                              int k, R *lds, Partial3 &p) {
+    using T = LdsTile<R, DY>;
+    const DMesh3<R> &m = A.m;
+    const size_t cnt = A.count;
+    const long long sy = m.ld, sz = m.plane, q = q0 + tx + (long long)sy * ty;
+    const R *w = A.in;
+    const R h = m.h, hinv = m.hinv;
+    constexpr int ord[9] = {0, 1, 2, 6, 3, 4, 7, 8, 5};
+    const int c0 = (ty + T::YLO) * T::W + tx + 4; // this node in the tile
+    auto stage = [&](int f, R *buf) {
+        const R *v = w + (size_t)f * cnt + q0 - 4 - (long long)T::YLO * sy;
+#pragma unroll
+        for (int e = threadIdx.x; e < T::N; e += THREADS) {
+            const int ry = e / T::W, rx = e - ry * T::W;
+            buf[e] = v[rx + (long long)ry * sy];
+        }
+    };
+    R mt[9];
+    R lambda = 0, mu = 0, density = 1;
+    if (active) {
+        lambda = mat(m, m.lam, q, k), mu = mat(m, m.mu, q, k), density = mat(m, m.rho, q, k);
+        if constexpr (BAND == TERRAIN) {
+#pragma unroll
+            for (int a = 0; a < 9; ++a) mt[a] = __builtin_nontemporal_load(m.met + (size_t)a * m.mcount + q);
+        }
+    }
+    auto comp = [&](const R d[3], int a) -> R {
+        if constexpr (BAND == FLAT) return d[a];
+        else return mt[a] * d[0] + mt[3 + a] * d[1] + mt[6 + a] * d[2];
+    };
+    R g[3][3], ctr[9], kv[9], d[3];
+    R g60 = 0, g61 = 0, m0 = 0, m1 = 0, g70 = 0, m2 = 0;
+    stage(ord[0], lds);
+    __syncthreads();
+#pragma unroll
+    for (int n = 0; n < 9; ++n) {
+        const int f = ord[n];
+        const R *cur = lds + (n & 1) * T::N;
+        if (n + 1 < 9) stage(ord[n + 1], lds + ((n + 1) & 1) * T::N);
+        if (active) {
+            ctr[f] = cur[c0];
+            R r = 0;
+#pragma unroll
+            for (int l = 0; l < 5; ++l) r += cf<R, DX>(l) * cur[c0 + of<DX>(l)];
+            d[0] = byh(r, h, hinv);
+            r = 0;
+#pragma unroll
+            for (int l = 0; l < 5; ++l) r += cf<R, DY>(l) * cur[c0 + of<DY>(l) * T::W];
+            d[1] = byh(r, h, hinv);
+            const R *v = w + (size_t)f * cnt;
+            r = 0;
+#pragma unroll
+            for (int l = 0; l < 5; ++l) r += cf<R, DZ>(l) * (of<DZ>(l) == 0 ? ctr[f] : v[q + of<DZ>(l) * sz]);
+            d[2] = byh(r, h, hinv);
+            if (n < 3) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) g[f][a] = comp(d, a);
+            } else if (f == 6) {
+                g60 = comp(d, 0), g61 = comp(d, 1);
+            } else if (f == 3) {
+                m0 = comp(d, 0) + g61;
+            } else if (f == 4) {
+                m1 = g60 + comp(d, 1);
+            } else if (f == 7) {
+                g70 = comp(d, 0);
+                kv[0] = (m0 + comp(d, 2)) / density;
+            } else if (f == 8) {
+                m2 = g70 + comp(d, 1);
+                kv[1] = (m1 + comp(d, 2)) / density;
+            } else { // f == 5
+                kv[2] = (m2 + comp(d, 2)) / density;
+            }
+        }
+        __syncthreads();
+    }
+    if (!active) return;
+    const R div = g[0][0] + g[1][1] + g[2][2];
+    kv[3] = lambda * div + 2 * mu * g[0][0];
+    kv[4] = lambda * div + 2 * mu * g[1][1];
+    kv[5] = lambda * div + 2 * mu * g[2][2];
+    kv[6] = mu * (g[0][1] + g[1][0]);
+    kv[7] = mu * (g[0][2] + g[2][0]);
+    kv[8] = mu * (g[1][2] + g[2][1]);
+    if (m.si1 >= 0 && i >= m.si0 && i <= m.si1 && j >= m.sj0 && j <= m.sj1 && k >= m.sk0 && k <= m.sk1) {
+        for (int a = 0; a < 3; ++a)
+            kv[a] += A.forcing * (m.grad[0] ? m.grad[a][q] : m.src[q] * m.dir[a]) * (m.rho_global / density);
+    }
+    const R damp = m.damp ? m.damp[q] : (m.dax[i] + m.day[j]) + m.daz[k];
+    R nxt[9];
+#pragma unroll
+    for (int f = 0; f < 9; ++f) {
+        const size_t o = (size_t)f * cnt + q;
+        const R kf = kv[f] - damp * ctr[f];
+        if (A.kind == 2) {
+            nxt[f] = __builtin_nontemporal_load(A.acc + o) + A.wy * kf;
+        } else {
+            const R b = __builtin_nontemporal_load(A.base + o);
+            nxt[f] = b + A.w * kf;
+            if constexpr (sizeof(R) == 4) {
+                if (A.kind == 1) {
+                    const R y = A.ystage ? b + (ctr[f] - b) * (R)(1.0 / 3.0) : __builtin_nontemporal_load(A.acc + o);
+                    __builtin_nontemporal_store(y + A.wy * kf, A.acc + o);
+                }
+            } else {
+                __builtin_nontemporal_store((A.kind == 0 ? b : __builtin_nontemporal_load(A.acc + o)) + A.wy * kf,
+                                            A.acc + o);
+            }
+        }
+        __builtin_nontemporal_store(nxt[f], A.out + o);
+    }
+    if (A.kind == 2) {
+        bool bad = false;
+        for (int f = 0; f < 9; ++f) bad = bad || !isfinite(nxt[f]);
+        const double vx = nxt[0], vy = nxt[1], vz = nxt[2];
+        const double v2 = vx * vx + vy * vy + vz * vz, sv = sqrt(v2);
+        p.velocity = sv;
+        p.peak[0] = fabs(vz), p.peak[1] = fabs(vx), p.peak[2] = fabs(vy), p.peak[3] = sv;
+        const double sxx = nxt[3], syy = nxt[4], szz = nxt[5], sxy = nxt[6], sxz = nxt[7], syz = nxt[8];
+        p.peak[4] = fabs((sxx + syy + szz) / 3.0);
+        const double dl = lambda, dm = mu, dr = density, mean = (sxx + syy + szz) / 3;
+        double dev = 0;
+        const double tt[3] = {sxx - mean, syy - mean, szz - mean};
+        for (int c = 0; c < 3; ++c) dev += tt[c] * tt[c];
+        dev += 2 * sxy * sxy, dev += 2 * sxz * sxz, dev += 2 * syz * syz;
+        const double bulk = dl + 2 * dm / 3, hh = (double)h;
+        const double jj = BAND == FLAT ? 1.0 : (double)m.jac[q];
+        p.energy = (0.5 * dr * v2 + dev / (4 * dm) + mean * mean / (2 * bulk)) * jj * hh * hh * hh;
+        if (!isfinite(v2) || !isfinite(p.energy)) bad = true;
+        p.bad = bad;
+    }
+}
+
+template <class R, int DX, int DY, int DZ, int BAND, bool LDS = false, int SD = -1>
 __global__ __launch_bounds__(THREADS) void stage3_kernel(Stage3<R> A) {
-    __shared__ Partial3 sh[THREADS];
     if (*A.status) return;
     const DMesh3<R> &m = A.m;
     // Each XCD takes a contiguous run of tile columns, planes varying fastest and then the y tile: the z- and
     // y-neighbours of a block's tile (the 3 + 1 halo planes and rows) are tiles its XCD's other resident blocks work
-    // on at the same time (L2 hits), instead of tiles a plane or a row of tiles later.  t is still the tile's
-    // plane-major index, so partials keep their order.
+    // on at the same time (L2 hits), instead of tiles a plane or a row of tiles later.  tl is the tile's
+    // plane-major index within the band (planes from A.k0), t its partial slot, so partials keep their order.
     const int tilesPerPlane = A.ntx * A.nty, u = tile_of_block(blockIdx.x, A.nband), planes = A.nband / tilesPerPlane;
-    const int t = A.t0 + (u % planes) * tilesPerPlane + u / planes;
-    if constexpr (BAND != SURFACE) {
-        const int per = A.ntx * A.nty, kk = t / per, rem = t % per;
-        const int i = 3 + (rem % A.ntx) * TX<R> + (threadIdx.x % TX<R>), j = 3 + (rem / A.ntx) * TY<R> + threadIdx.x / TX<R>, k = 3 + kk;
+    const int tl = tile_in_band(u, planes, A.ntx, A.nty, A.bx, A.by, A.bz), t = A.t0 + tl;
+    if constexpr (BAND != SURFACE && LDS) {
+        __shared__ R lds[2 * LdsTile<R, DY>::N];
+        __shared__ Partial3 sh4[THREADS / 64];
+        const int per = A.ntx * A.nty, kk = tl / per, rem = tl % per;
+        const int tx = threadIdx.x % TX<R>, ty = threadIdx.x / TX<R>, i0 = 3 + (rem % A.ntx) * TX<R>, j0 = 3 + (rem / A.ntx) * TY<R>;
+        const int i = i0 + tx, j = j0 + ty, k = A.k0 + kk;
+        const bool active = i < m.nx - 3 && j < m.ny - 3;
+        Partial3 p{};
+        deep_node_lds<R, DX, DY, DZ, BAND>(A, at3(m, i0, j0, k), tx, ty, active, i, j, k, lds, p);
+        if (A.kind == 2) {
+            block_reduce3_waves(p, sh4);
+            if (threadIdx.x == 0) A.partials[t] = p;
+        }
+        return;
+    }
+    if constexpr (BAND != SURFACE && !LDS) {
+        __shared__ Partial3 sh[THREADS];
+        const int per = A.ntx * A.nty, kk = tl / per, rem = tl % per;
+        const int i = 3 + (rem % A.ntx) * TX<R> + (threadIdx.x % TX<R>), j = 3 + (rem / A.ntx) * TY<R> + threadIdx.x / TX<R>, k = A.k0 + kk;
         Partial3 p{};
         if (i < m.nx - 3 && j < m.ny - 3) deep_node<R, DX, DY, DZ, BAND>(A, at3(m, i, j, k), i, j, k, p);
         if (A.kind == 2) {
@@ -349,15 +549,16 @@ __global__ __launch_bounds__(THREADS) void stage3_kernel(Stage3<R> A) {
         }
         return;
     }
-    const int per = A.ntx * A.nty, kk = t / per, rem = t % per;
-    const int i = 3 + (rem % A.ntx) * TX<R> + (threadIdx.x % TX<R>), j = 3 + (rem / A.ntx) * TY<R> + threadIdx.x / TX<R>, k = 3 + kk;
+    __shared__ Partial3 sh[THREADS];
+    const int per = A.ntx * A.nty, kk = tl / per, rem = tl % per;
+    const int i = 3 + (rem % A.ntx) * TX<R> + (threadIdx.x % TX<R>), j = 3 + (rem / A.ntx) * TY<R> + threadIdx.x / TX<R>, k = A.k0 + kk;
     Partial3 p{};
     if (i < m.nx - 3 && j < m.ny - 3) {
         const size_t cnt = A.count;
         const long long q = at3(m, i, j, k), sy = m.ld, sz = m.plane;
         const R *w = A.in;
         const R h = m.h, hinv = m.hinv;
-        const int depth = k - 3;
+        const int depth = SD >= 0 ? SD : k - 3; // SD >= 0: a launch over the one surface plane at that depth
         const R lambda = mat(m, m.lam, q, k), mu = mat(m, m.mu, q, k), density = mat(m, m.rho, q, k);
         double dzs[3] = {0, 0, 0};
         if (depth == 0) closure<R, DX, DY>(m, w, cnt, q, i, j, dzs);
@@ -456,6 +657,239 @@ __global__ __launch_bounds__(THREADS) void stage3_kernel(Stage3<R> A) {
     if (A.kind == 2) {
         block_reduce3(p, sh);
         if (threadIdx.x == 0) A.partials[t] = sh[0];
+    }
+}
+
+// ---- deep bands with vector lanes: each lane owns NV x-adjacent nodes of one row (one vector of NV values), the
+// LPR lanes of a 128-byte row segment and a block THREADS / LPR rows.  The x taps come from the aligned vectors
+// around the lane's own, the y and z taps are whole aligned vectors: a field costs 11-12 vector loads per NV nodes
+// instead of 13 scalar loads per node.  TERRAIN reads the lane's 9 metrics once, as vectors; FLAT has identity
+// metrics.  Every node's arithmetic is deep_node's, term for term, so the results are the same to the bit as the
+// one-node-per-lane body.
+template <class R, int NV> struct VLane3 {
+    typedef R V __attribute__((ext_vector_type(NV)));
+};
+template <class R, int NV> __device__ inline typename VLane3<R, NV>::V vld3(const R *p) {
+    return *reinterpret_cast<const typename VLane3<R, NV>::V *>(p);
+}
+template <class R, int NV> __device__ inline typename VLane3<R, NV>::V vld3_nt(const R *p) {
+    return __builtin_nontemporal_load(reinterpret_cast<const typename VLane3<R, NV>::V *>(p));
+}
+template <class R, int NV> __device__ inline void vst3_nt(R *p, typename VLane3<R, NV>::V v) {
+    __builtin_nontemporal_store(v, reinterpret_cast<typename VLane3<R, NV>::V *>(p));
+}
+
+template <class R, int DX, int DY, int DZ, int NV, int BAND>
+__global__ __launch_bounds__(THREADS) void band_vec_kernel(Stage3<R> A) {
+    using V = typename VLane3<R, NV>::V;
+    constexpr int LPR = 128 / (NV * (int)sizeof(R)), VTX = LPR * NV, VROWS = THREADS / LPR; // 128-byte row segments
+    // x window: the aligned vectors holding offsets -1..3 (DX = 0) or -3..1 (DX = 1) of every node of the lane
+    constexpr int XLO = DX ? -3 : -1, XHI = DX ? 1 : 3;
+    constexpr int WLO = (XLO - (NV - 1)) / NV, WHI = (NV - 1 + XHI) / NV; // floor division (XLO < 0)
+    __shared__ Partial3 sh4[THREADS / 64];
+    if (*A.status) return;
+    const DMesh3<R> &m = A.m;
+    const int per = A.ntx * A.nty, u = tile_of_block(blockIdx.x, A.nband), planes = A.nband / per;
+    const int tv = tile_in_band(u, planes, A.ntx, A.nty, A.bx, A.by, A.bz), kk = tv / per, rem = tv % per;
+    const int i0 = 3 + (rem % A.ntx) * VTX + (threadIdx.x % LPR) * NV, j = 3 + (rem / A.ntx) * VROWS + threadIdx.x / LPR;
+    const int k = A.k0 + kk;
+    Partial3 p{};
+    if (i0 < m.nx - 3 && j < m.ny - 3) {
+        const int valid = min(NV, m.nx - 3 - i0); // nodes of the lane inside the physical grid
+        const size_t cnt = A.count;
+        const long long q = at3(m, i0, j, k), sy = m.ld, sz = m.plane;
+        const R h = m.h, hinv = m.hinv;
+        R lambda[NV], mu[NV], density[NV];
+        if (m.by_depth) {
+#pragma unroll
+            for (int n = 0; n < NV; ++n) lambda[n] = m.lam[k], mu[n] = m.mu[k], density[n] = m.rho[k];
+        } else {
+            const V l = vld3<R, NV>(m.lam + q), u2 = vld3<R, NV>(m.mu + q), r = vld3<R, NV>(m.rho + q);
+#pragma unroll
+            for (int n = 0; n < NV; ++n) lambda[n] = l[n], mu[n] = u2[n], density[n] = r[n];
+        }
+        R mt[BAND == TERRAIN ? 9 : 1][NV];
+        if constexpr (BAND == TERRAIN) {
+#pragma unroll
+            for (int a = 0; a < 9; ++a) { // read once
+                const V x = vld3_nt<R, NV>(m.met + (size_t)a * m.mcount + q);
+#pragma unroll
+                for (int n = 0; n < NV; ++n) mt[a][n] = x[n];
+            }
+        }
+        // Cartesian component a of a derivative triple at node n: identity metrics, or deep_node's comp()
+        auto cmp = [&](const R (&d)[3][NV], int a, int n) -> R {
+            if constexpr (BAND == FLAT) return d[a][n];
+            else return mt[a][n] * d[0][n] + mt[3 + a][n] * d[1][n] + mt[6 + a][n] * d[2][n];
+        };
+        // DRP derivatives of field f along x, y, z at the lane's NV nodes (drp's sum order for each node)
+        auto deriv = [&](int f, R d[3][NV]) {
+            const R *v = A.in + (size_t)f * cnt;
+            R win[(WHI - WLO + 1) * NV];
+#pragma unroll
+            for (int b = WLO; b <= WHI; ++b) {
+                const V x = vld3<R, NV>(v + q + b * NV);
+#pragma unroll
+                for (int n = 0; n < NV; ++n) win[(b - WLO) * NV + n] = x[n];
+            }
+#pragma unroll
+            for (int n = 0; n < NV; ++n) {
+                R r = 0;
+#pragma unroll
+                for (int l = 0; l < 5; ++l) r += cf<R, DX>(l) * win[-WLO * NV + n + of<DX>(l)];
+                d[0][n] = byh(r, h, hinv);
+            }
+            R ry[NV], rz[NV];
+#pragma unroll
+            for (int n = 0; n < NV; ++n) ry[n] = 0, rz[n] = 0;
+#pragma unroll
+            for (int l = 0; l < 5; ++l) {
+                const V a = of<DY>(l) == 0 ? V{} : vld3<R, NV>(v + q + of<DY>(l) * sy);
+                const V b = of<DZ>(l) == 0 ? V{} : vld3<R, NV>(v + q + of<DZ>(l) * sz);
+#pragma unroll
+                for (int n = 0; n < NV; ++n) {
+                    ry[n] += cf<R, DY>(l) * (of<DY>(l) == 0 ? win[-WLO * NV + n] : a[n]);
+                    rz[n] += cf<R, DZ>(l) * (of<DZ>(l) == 0 ? win[-WLO * NV + n] : b[n]);
+                }
+            }
+#pragma unroll
+            for (int n = 0; n < NV; ++n) d[1][n] = byh(ry[n], h, hinv), d[2][n] = byh(rz[n], h, hinv);
+        };
+        R d[3][NV], kv[9][NV];
+        {
+            R g[3][3][NV];
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                deriv(f, d);
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int n = 0; n < NV; ++n) g[f][a][n] = cmp(d, a, n);
+            }
+#pragma unroll
+            for (int n = 0; n < NV; ++n) {
+                const R div = g[0][0][n] + g[1][1][n] + g[2][2][n];
+                kv[3][n] = lambda[n] * div + 2 * mu[n] * g[0][0][n];
+                kv[4][n] = lambda[n] * div + 2 * mu[n] * g[1][1][n];
+                kv[5][n] = lambda[n] * div + 2 * mu[n] * g[2][2][n];
+                kv[6][n] = mu[n] * (g[0][1][n] + g[1][0][n]);
+                kv[7][n] = mu[n] * (g[0][2][n] + g[2][0][n]);
+                kv[8][n] = mu[n] * (g[1][2][n] + g[2][1][n]);
+            }
+        }
+        // momentum, in deep_node's order
+        R g60[NV], m0[NV], m1[NV], g70[NV], m2[NV];
+        deriv(6, d);
+#pragma unroll
+        for (int n = 0; n < NV; ++n) g60[n] = cmp(d, 0, n), m0[n] = cmp(d, 1, n); // m0 holds g61 until field 3
+        deriv(3, d);
+#pragma unroll
+        for (int n = 0; n < NV; ++n) m0[n] = cmp(d, 0, n) + m0[n];
+        deriv(4, d);
+#pragma unroll
+        for (int n = 0; n < NV; ++n) m1[n] = g60[n] + cmp(d, 1, n);
+        deriv(7, d);
+#pragma unroll
+        for (int n = 0; n < NV; ++n) g70[n] = cmp(d, 0, n), kv[0][n] = (m0[n] + cmp(d, 2, n)) / density[n];
+        deriv(8, d);
+#pragma unroll
+        for (int n = 0; n < NV; ++n) m2[n] = g70[n] + cmp(d, 1, n), kv[1][n] = (m1[n] + cmp(d, 2, n)) / density[n];
+        deriv(5, d);
+#pragma unroll
+        for (int n = 0; n < NV; ++n) kv[2][n] = (m2[n] + cmp(d, 2, n)) / density[n];
+        if (m.si1 >= 0 && j >= m.sj0 && j <= m.sj1 && k >= m.sk0 && k <= m.sk1) {
+#pragma unroll
+            for (int n = 0; n < NV; ++n) {
+                const int i = i0 + n;
+                if (i < m.si0 || i > m.si1) continue;
+                for (int a = 0; a < 3; ++a)
+                    kv[a][n] += A.forcing * (m.grad[0] ? m.grad[a][q + n] : m.src[q + n] * m.dir[a]) * (m.rho_global / density[n]);
+            }
+        }
+        R damp[NV];
+        if (m.damp) {
+            const V dv = vld3<R, NV>(m.damp + q);
+#pragma unroll
+            for (int n = 0; n < NV; ++n) damp[n] = dv[n];
+        } else {
+#pragma unroll
+            for (int n = 0; n < NV; ++n) damp[n] = (m.dax[i0 + n] + m.day[j]) + m.daz[k];
+        }
+        R nxt[9][NV];
+#pragma unroll
+        for (int f = 0; f < 9; ++f) {
+            const size_t o = (size_t)f * cnt + q;
+            const V c = vld3<R, NV>(A.in + o);
+            V kf;
+#pragma unroll
+            for (int n = 0; n < NV; ++n) kf[n] = kv[f][n] - damp[n] * c[n];
+            V out, accv;
+            if (A.kind == 2) {
+                const V y = vld3_nt<R, NV>(A.acc + o);
+#pragma unroll
+                for (int n = 0; n < NV; ++n) out[n] = y[n] + A.wy * kf[n];
+            } else {
+                const V b = vld3_nt<R, NV>(A.base + o);
+#pragma unroll
+                for (int n = 0; n < NV; ++n) out[n] = b[n] + A.w * kf[n];
+                bool storeAcc = true;
+                if constexpr (sizeof(R) == 4) {
+                    if (A.kind == 1) {
+                        const V y = A.ystage ? V{} : vld3_nt<R, NV>(A.acc + o);
+#pragma unroll
+                        for (int n = 0; n < NV; ++n)
+                            accv[n] = (A.ystage ? b[n] + (c[n] - b[n]) * (R)(1.0 / 3.0) : y[n]) + A.wy * kf[n];
+                    } else {
+                        storeAcc = false; // stage 1: Y1 is rebuilt at stage 2
+                    }
+                } else {
+                    const V y = A.kind == 0 ? b : vld3_nt<R, NV>(A.acc + o);
+#pragma unroll
+                    for (int n = 0; n < NV; ++n) accv[n] = y[n] + A.wy * kf[n];
+                }
+                if (storeAcc) {
+                    if (valid == NV) vst3_nt<R, NV>(A.acc + o, accv);
+                    else
+                        for (int n = 0; n < valid; ++n) A.acc[o + n] = accv[n];
+                }
+            }
+            if (valid == NV) vst3_nt<R, NV>(A.out + o, out);
+            else
+                for (int n = 0; n < valid; ++n) A.out[o + n] = out[n];
+#pragma unroll
+            for (int n = 0; n < NV; ++n) nxt[f][n] = out[n];
+        }
+        if (A.kind == 2) {
+#pragma unroll
+            for (int n = 0; n < NV; ++n) {
+                if (n >= valid) break;
+                Partial3 e{};
+                bool bad = false;
+                for (int f = 0; f < 9; ++f) bad = bad || !isfinite(nxt[f][n]);
+                const double vx = nxt[0][n], vy = nxt[1][n], vz = nxt[2][n];
+                const double v2 = vx * vx + vy * vy + vz * vz, sv = sqrt(v2);
+                e.velocity = sv;
+                e.peak[0] = fabs(vz), e.peak[1] = fabs(vx), e.peak[2] = fabs(vy), e.peak[3] = sv;
+                const double sxx = nxt[3][n], syy = nxt[4][n], szz = nxt[5][n], sxy = nxt[6][n], sxz = nxt[7][n],
+                             syz = nxt[8][n];
+                e.peak[4] = fabs((sxx + syy + szz) / 3.0);
+                const double dl = lambda[n], dm = mu[n], dr = density[n], mean = (sxx + syy + szz) / 3;
+                double dev = 0;
+                const double tt[3] = {sxx - mean, syy - mean, szz - mean};
+                for (int c = 0; c < 3; ++c) dev += tt[c] * tt[c];
+                dev += 2 * sxy * sxy, dev += 2 * sxz * sxz, dev += 2 * syz * syz;
+                const double bulk = dl + 2 * dm / 3, hh = (double)h;
+                const double jj = BAND == FLAT ? 1.0 : (double)m.jac[q + n];
+                e.energy = (0.5 * dr * v2 + dev / (4 * dm) + mean * mean / (2 * bulk)) * jj * hh * hh * hh;
+                if (!isfinite(v2) || !isfinite(e.energy)) bad = true;
+                e.bad = bad;
+                merge3(p, e);
+            }
+        }
+    }
+    if (A.kind == 2) {
+        block_reduce3_waves(p, sh4);
+        if (threadIdx.x == 0) A.partials[A.t0 + tv] = p;
     }
 }
 

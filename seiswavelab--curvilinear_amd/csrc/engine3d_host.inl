@@ -6,10 +6,13 @@ namespace swl {
 namespace {
 
 template <class R> struct Engine3D {
+    static constexpr int real_bytes = sizeof(R);
     int device = 0, nrec = 0;
     int nx = 0, ny = 0, nz = 0, ld = 0, off = ORIGIN;
     size_t count = 0;
     hipStream_t stream{};
+    hipStream_t side{};                // the surface band, concurrent with the deep bands of the same stage
+    hipEvent_t fork{}, join{};
     DMesh3<R> m{};
     DeviceBuffer met, jac, lam, mu, rho, dax, day, daz, damp, src, grad[3], smet, slam, smu, snrm;
     DeviceBuffer state[3], accbuf, partials[2], level2, probes, status, accepted, peak, speak, last, diag, samples,
@@ -17,6 +20,14 @@ template <class R> struct Engine3D {
     int ntx = 0, nty = 0, ntiles = 0, nLevel2 = 0, cur = 0, prev = 0, curPart = 0, prevPart = 0, registers = 0;
     int diagRows = 0;
     bool profiling = false;
+    int ldsTiles = 0; // deep bands: LDS-staged stencil input (option "lds_tiles")
+    int brick[3] = {0, 0, 0}; // tile order: bricks of x by y tiles by z planes per XCD run (0: tile columns)
+    // deep bands with this many x-adjacent nodes per lane, as one vector (options "vector_terrain", "vector_flat":
+    // 2, or 4 in FP32; 0 / 1: one node per lane)
+    int vectorTerrain = 2, vectorFlat = 16 / (int)sizeof(R);
+    int splitSurface = 0; // surface band: one launch per plane (option "split_surface"; 0: one launch for 4 planes)
+    int overlapBands = 1; // surface band on the side stream beside the deep bands (option "overlap_bands")
+    int written[2] = {0, 0}; // partial slots the last step wrote into partials[0], partials[1]
     std::vector<hipEvent_t> events;
     size_t eventsUsed = 0;
     double stageMs = 0;
@@ -69,6 +80,9 @@ template <class R> struct Engine3D {
         ld = (nx + 63) / 64 * 64;
         count = ((size_t)off + (size_t)ld * ny * nz + 63) / 64 * 64;
         check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+        check(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), "stream");
+        check(hipEventCreateWithFlags(&fork, hipEventDisableTiming), "event");
+        check(hipEventCreateWithFlags(&join, hipEventDisableTiming), "event");
         const size_t N = (size_t)nx * ny * nz;
         m.nx = nx, m.ny = ny, m.nz = nz, m.ld = ld, m.off = off, m.plane = (long long)ld * ny;
         m.h = (R)d.h;
@@ -166,7 +180,9 @@ template <class R> struct Engine3D {
         accbuf.allocate((size_t)9 * count * sizeof(R));
         ntx = (nx - 6 + TX<R> - 1) / TX<R>, nty = (ny - 6 + TY<R> - 1) / TY<R>;
         ntiles = ntx * nty * (nz - 6);
-        for (auto &b : partials) b.allocate((size_t)ntiles * sizeof(Partial3));
+        // vector bands write one partial per (larger) tile: at most as many slots as one node per lane
+        for (auto &b : partials) b.allocate((size_t)std::max(ntiles, partialSlots(1, 1)) * sizeof(Partial3));
+        written[0] = written[1] = ntiles;
         nLevel2 = std::min(128, std::max(1, ntiles / 64));
         level2.allocate((size_t)nLevel2 * sizeof(Partial3));
         std::vector<long long> pi(3 * (size_t)std::max(n, 1));
@@ -188,6 +204,9 @@ template <class R> struct Engine3D {
     }
     ~Engine3D() {
         for (auto e : events) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -196,16 +215,78 @@ template <class R> struct Engine3D {
         return (flags[step % 8][axis] + stage) % 2;
     }
     // planes [3, 7) surface band, [7, flat) terrain band, [max(7, flat), nz - 3) flat band
-    template <int DX, int DY, int DZ> void launch(Stage3<R> a) {
-        const int per = ntx * nty, kEnd = nz - 3, kSurf = std::min(7, kEnd), kFlat = std::clamp(m.flat, kSurf, kEnd);
-        const int bands[3][2] = {{3, kSurf}, {kSurf, kFlat}, {kFlat, kEnd}};
-        for (int b = 0; b < 3; ++b) {
-            if (bands[b][1] <= bands[b][0]) continue;
-            a.t0 = per * (bands[b][0] - 3), a.nband = per * (bands[b][1] - bands[b][0]);
-            if (b == 0) stage3_kernel<R, DX, DY, DZ, SURFACE><<<a.nband, THREADS, 0, stream>>>(a);
-            else if (b == 1) stage3_kernel<R, DX, DY, DZ, TERRAIN><<<a.nband, THREADS, 0, stream>>>(a);
-            else stage3_kernel<R, DX, DY, DZ, FLAT><<<a.nband, THREADS, 0, stream>>>(a);
+    int flatFrom() const { return std::clamp(m.flat, std::min(7, nz - 3), nz - 3); }
+    // tiles per plane of a deep band: TX x TY nodes, or with nv nodes per lane 128-byte row segments by
+    // THREADS / (128 / (nv * sizeof(R))) rows
+    void bandTiles(int nv, int &tx, int &ty) const {
+        if (nv <= 1) {
+            tx = ntx, ty = nty;
+            return;
         }
+        const int lanes = 128 / (nv * (int)sizeof(R)), w = lanes * nv, h = THREADS / lanes;
+        tx = (nx - 6 + w - 1) / w, ty = (ny - 6 + h - 1) / h;
+    }
+    // band b (0 surface, 1 terrain, 2 flat): planes [k0, k1), first partial slot t0, tiles per plane
+    struct Band {
+        int k0, k1, t0, tx, ty;
+    };
+    Band band(int b, int vterrain, int vflat) const {
+        const int kEnd = nz - 3, kSurf = std::min(7, kEnd), kFlat = flatFrom();
+        const int k[4] = {3, kSurf, kFlat, kEnd};
+        const int vec[3] = {0, vterrain, vflat};
+        Band r{k[b], k[b + 1], 0, 0, 0};
+        for (int c = 0; c <= b; ++c) {
+            int tx, ty;
+            bandTiles(vec[c], tx, ty);
+            if (c < b) r.t0 += tx * ty * (k[c + 1] - k[c]);
+            else r.tx = tx, r.ty = ty;
+        }
+        return r;
+    }
+    // partial slots of a step: one per tile of the three bands
+    int partialSlots(int vterrain, int vflat) const {
+        const Band f = band(2, vterrain, vflat);
+        return f.t0 + f.tx * f.ty * (f.k1 - f.k0);
+    }
+    // The bands of a stage write disjoint planes and partial slots and only read the stage input, so with
+    // overlapBands the surface band (few, long-latency blocks) runs on the side stream beside the deep bands;
+    // the next launch on the engine stream waits for both.
+    template <int DX, int DY, int DZ> void launch(Stage3<R> a) {
+        bool forked = false;
+        for (int b = 0; b < 3; ++b) {
+            const Band d = band(b, vectorTerrain, vectorFlat);
+            if (d.k1 <= d.k0) continue;
+            a.t0 = d.t0, a.ntx = d.tx, a.nty = d.ty, a.nband = d.tx * d.ty * (d.k1 - d.k0), a.k0 = d.k0;
+            hipStream_t stream = this->stream;
+            if (b == 0 && overlapBands && d.k1 < nz - 3) {
+                check(hipEventRecord(fork, stream), "3D band fork");
+                check(hipStreamWaitEvent(side, fork, 0), "3D band fork");
+                stream = side, forked = true;
+            }
+            if (b == 0 && splitSurface) {
+                // one launch per surface plane, each compiled for its depth (fewer live registers than one body)
+                const int per = d.tx * d.ty;
+                Stage3<R> s = a;
+                s.nband = per;
+                for (int depth = 0; 3 + depth < d.k1; ++depth) {
+                    s.k0 = 3 + depth, s.t0 = d.t0 + per * depth;
+                    if (depth == 0) stage3_kernel<R, DX, DY, DZ, SURFACE, false, 0><<<per, THREADS, 0, stream>>>(s);
+                    else if (depth == 1) stage3_kernel<R, DX, DY, DZ, SURFACE, false, 1><<<per, THREADS, 0, stream>>>(s);
+                    else if (depth == 2) stage3_kernel<R, DX, DY, DZ, SURFACE, false, 2><<<per, THREADS, 0, stream>>>(s);
+                    else stage3_kernel<R, DX, DY, DZ, SURFACE, false, 3><<<per, THREADS, 0, stream>>>(s);
+                }
+            } else if (b == 0) stage3_kernel<R, DX, DY, DZ, SURFACE><<<a.nband, THREADS, 0, stream>>>(a);
+            else if (b == 1 && vectorTerrain == 4) band_vec_kernel<R, DX, DY, DZ, 16 / sizeof(R), TERRAIN><<<a.nband, THREADS, 0, stream>>>(a);
+            else if (b == 1 && vectorTerrain == 2) band_vec_kernel<R, DX, DY, DZ, 2, TERRAIN><<<a.nband, THREADS, 0, stream>>>(a);
+            else if (b == 1 && a.lds) stage3_kernel<R, DX, DY, DZ, TERRAIN, true><<<a.nband, THREADS, 0, stream>>>(a);
+            else if (b == 1) stage3_kernel<R, DX, DY, DZ, TERRAIN><<<a.nband, THREADS, 0, stream>>>(a);
+            else if (vectorFlat == 4) band_vec_kernel<R, DX, DY, DZ, 16 / sizeof(R), FLAT><<<a.nband, THREADS, 0, stream>>>(a);
+            else if (vectorFlat == 2) band_vec_kernel<R, DX, DY, DZ, 2, FLAT><<<a.nband, THREADS, 0, stream>>>(a);
+            else if (a.lds) stage3_kernel<R, DX, DY, DZ, FLAT, true><<<a.nband, THREADS, 0, stream>>>(a);
+            else stage3_kernel<R, DX, DY, DZ, FLAT><<<a.nband, THREADS, 0, stream>>>(a);
+            if (b == 0 && forked) check(hipEventRecord(join, side), "3D band join");
+        }
+        if (forked) check(hipStreamWaitEvent(stream, join, 0), "3D band join");
     }
     void launch_stage(int dx, int dy, int dz, const Stage3<R> &a) {
         switch (dx * 4 + dy * 2 + dz) {
@@ -240,8 +321,11 @@ template <class R> struct Engine3D {
             a.status = status.as<int>();
             a.partials = partials[part].as<Partial3>();
             a.ntx = ntx, a.nty = nty, a.ntiles = ntiles;
+            a.lds = ldsTiles;
+            a.bx = brick[0], a.by = brick[1], a.bz = brick[2];
             launch_stage(direction(step, s, 0), direction(step, s, 1), direction(step, s, 2), a);
         }
+        written[part] = partialSlots(vectorTerrain, vectorFlat);
         mark();
         return b2;
     }
@@ -258,7 +342,7 @@ template <class R> struct Engine3D {
         return f;
     }
     void reduce(int part) {
-        reduce3_kernel<<<nLevel2, THREADS, 0, stream>>>(partials[part].as<Partial3>(), ntiles, level2.as<Partial3>(),
+        reduce3_kernel<<<nLevel2, THREADS, 0, stream>>>(partials[part].as<Partial3>(), written[part], level2.as<Partial3>(),
                                                          status.as<int>());
         check(hipGetLastError(), "3D reduce launch");
     }
@@ -464,6 +548,26 @@ int swl_engine3d_profile(swl_engine3d *e, int enable) {
         e->with([&](auto &x) {
             x.profiling = enable != 0;
             x.stageMs = 0, x.stageLaunches = 0, x.eventsUsed = 0;
+        });
+    });
+}
+int swl_engine3d_set_option(swl_engine3d *e, const char *name, int value) {
+    return swl::guarded([&] {
+        const std::string key = name ? name : "";
+        e->with([&](auto &x) {
+            if (key == "lds_tiles") x.ldsTiles = value != 0;
+            else if (key == "brick_x") x.brick[0] = std::max(0, value);
+            else if (key == "brick_y") x.brick[1] = std::max(1, value);
+            else if (key == "brick_z") x.brick[2] = std::max(1, value);
+            else if (key == "split_surface") x.splitSurface = value != 0;
+            else if (key == "overlap_bands") x.overlapBands = value != 0;
+            else if (key == "vector_flat" || key == "vector_terrain") {
+                using T = std::remove_reference_t<decltype(x)>;
+                swl::require(value == 0 || value == 1 || value == 2 || (value == 4 && T::real_bytes == 4),
+                             "vector_flat / vector_terrain take 0 or 1 (one node per lane), 2, or 4 (FP32)");
+                (key == "vector_flat" ? x.vectorFlat : x.vectorTerrain) = value;
+            }
+            else throw std::runtime_error("unknown engine option: " + key);
         });
     });
 }

@@ -144,6 +144,12 @@ int swl_sim3d_profile(swl_sim3d *s, int enable) {
         if (swl_engine3d_profile(s->engine, enable)) throw std::runtime_error(swl::last_error());
     });
 }
+int swl_sim3d_set_option(swl_sim3d *s, const char *name, int value) {
+    return swl::guarded([&] {
+        swl::require(s->engine != nullptr, "no device session");
+        if (swl_engine3d_set_option(s->engine, name, value)) throw std::runtime_error(swl::last_error());
+    });
+}
 int swl_sim3d_kernel_timing(const swl_sim3d *s, double *out) {
     return swl::guarded([&] {
         swl::require(s->engine != nullptr, "no device session");

@@ -177,6 +177,8 @@ SIGNATURES = {
     "swl_sim3d_read_display_frame": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]),
     "swl_sim3d_profile": (C.c_int, [vp, C.c_int]),
     "swl_sim3d_kernel_timing": (C.c_int, [vp, vp]),
+    "swl_sim3d_set_option": (C.c_int, [vp, C.c_char_p, C.c_int]),
+    "swl_engine3d_set_option": (C.c_int, [vp, C.c_char_p, C.c_int]),
 }
 
 

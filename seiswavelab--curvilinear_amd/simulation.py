@@ -554,6 +554,12 @@ class Simulation3D:
         self._need()
         native.call("swl_sim3d_profile", self._h, int(enable))
 
+    def set_option(self, name: str, value: int):
+        """Engine tuning switches: "vector_terrain", "vector_flat", "overlap_bands", "split_surface", "lds_tiles",
+        "brick_x", "brick_y", "brick_z" (results identical; see swl_engine3d_set_option)."""
+        self._need()
+        native.call("swl_sim3d_set_option", self._h, name.encode(), int(value))
+
     def override_source_peak(self, peak: float):
         """Replaces the source-window peak energy of the late-growth fault rule (CurvilinearSolver.cpp:477-488)."""
         self._need()

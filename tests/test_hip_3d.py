@@ -183,3 +183,63 @@ def test_device_geometry_dem_workload_3d():
         assert np.array_equal(dev.model_array(which), host.model_array(which)), which
     print(f"[3d geometry] host plan {1e3 * t_host:.0f} ms, HIP session (geometry on the GPU, engine set-up, "
           f"upload) {dd['initializationMs']:.0f} ms", flush=True)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("name", ["dem_layers_moment", "flat_datum_layers"])
+def test_lds_tiles_equal_direct_loads_3d(name, precision):
+    """The deep bands' LDS-staged stencil input (option "lds_tiles") reads the same values as direct loads, so
+    fields and traces are bit-identical either way."""
+    p = case(name)
+    runs = []
+    for tiles in (0, 1):
+        s = _gpu(p, precision)
+        s.set_option("lds_tiles", tiles)
+        ok, err = s.advance(p.maxSteps)
+        assert ok, err
+        runs.append((s.receiver_arrays(), [s.field(f) for f in range(9)]))
+        s.close()
+    (ra, fa), (rb, fb) = runs
+    for k in ("vx", "vy", "vz"):
+        assert np.array_equal(ra[k], rb[k]), k
+    for f in range(9):
+        assert np.array_equal(fa[f], fb[f]), f
+
+
+def _odd_case(name):
+    """A case whose flat band has ragged edges: 27 physical columns (not a multiple of 4 or 2 nodes per lane) and
+    37 rows (two 32-row vector tiles), the source inside the flat band."""
+    p = case(name)
+    p.nx, p.ny, p.nz, p.maxSteps = 27, 37, 26, 16
+    p.sourceX, p.sourceY, p.sourceZ = 13, 17, 12
+    return p
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("name", ["flat_datum_layers", "analytic_sponge_force"])
+def test_vector_bands_equal_node_lanes_3d(name, precision):
+    """The deep bands with 2 or 4 x-adjacent nodes per lane (options "vector_terrain", "vector_flat") compute every
+    node with the one-node-per-lane expressions: fields and traces bit-identical in every combination; the energy
+    (a sum over tiles of another shape) to 1e-12.  analytic_sponge_force has a terrain band (planes 7..14) and a
+    flat band."""
+    p = _odd_case(name)
+    runs = []
+    widths = ((0, 0), (2, 2), (2, 0), (0, 2)) if precision == 0 else ((0, 0), (2, 4), (4, 2), (4, 0), (0, 4))
+    for terrain, flat in widths:
+        s = _gpu(p, precision)
+        s.set_option("vector_terrain", terrain)
+        s.set_option("vector_flat", flat)
+        t = s.kernel_timing()
+        assert max(7, t["metric_planes"]) <= p.nz + 3 - 4, "the case has no flat band of 4 planes"
+        ok, err = s.advance(p.maxSteps)
+        assert ok, err
+        runs.append((s.receiver_arrays(), [s.field(f) for f in range(9)], s.diagnostics()))
+        s.close()
+    ra, fa, da = runs[0]
+    for rb, fb, db in runs[1:]:
+        for k in ("vx", "vy", "vz"):
+            assert np.array_equal(ra[k], rb[k]), k
+        for f in range(9):
+            assert np.array_equal(fa[f], fb[f]), f
+        assert abs(db["elasticEnergy"] - da["elasticEnergy"]) <= 1e-12 * da["elasticEnergy"]
+        assert db["maxAbsVelocity"] == da["maxAbsVelocity"] and db["allFinite"] == da["allFinite"]
