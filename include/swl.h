@@ -449,7 +449,8 @@ typedef struct swl_run3d {
 int swl_engine3d_run(swl_engine3d *e, int first_step, int count, double dt, double frequency, double scale,
                      double source_peak, double *diag, double *samples, swl_run3d *result);
 /* HIP-event timing of the stage kernels (one event pair per step), as for 2D. out[6]: stage ms,
-   stage launches, nodes per launch, VGPRs, metric planes stored, material by depth (0/1). */
+   stage launches, nodes per launch, VGPRs of the flat band's kernel, metric planes stored, material by depth
+   (0/1). */
 int swl_engine3d_profile(swl_engine3d *e, int enable);
 int swl_engine3d_kernel_timing(swl_engine3d *e, double *out);
 /* Engine tuning switches, all with identical fields (the energy sum may differ in its last bits, summation order):

@@ -17,7 +17,7 @@ template <class R> struct Engine3D {
     DeviceBuffer met, jac, lam, mu, rho, dax, day, daz, damp, src, grad[3], smet, slam, smu, snrm;
     DeviceBuffer state[3], accbuf, partials[2], level2, probes, status, accepted, peak, speak, last, diag, samples,
         gather, slice, plain;
-    int ntx = 0, nty = 0, ntiles = 0, nLevel2 = 0, cur = 0, prev = 0, curPart = 0, prevPart = 0, registers = 0;
+    int ntx = 0, nty = 0, ntiles = 0, nLevel2 = 0, cur = 0, prev = 0, curPart = 0, prevPart = 0;
     int diagRows = 0;
     bool profiling = false;
     int ldsTiles = 0; // deep bands: LDS-staged stencil input (option "lds_tiles")
@@ -197,9 +197,6 @@ template <class R> struct Engine3D {
         gather.allocate((size_t)std::max(n, 1) * 3 * sizeof(double));
         const int pl = std::max({(nx - 6) * (nz - 6), (ny - 6) * (nz - 6), (nx - 6) * (ny - 6)});
         slice.allocate((size_t)pl * sizeof(float));
-        hipFuncAttributes attr{};
-        if (hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&stage3_kernel<R, 0, 0, 0, FLAT>)) == hipSuccess)
-            registers = attr.numRegs;
         check(hipStreamSynchronize(stream), "3D engine setup");
     }
     ~Engine3D() {
@@ -216,6 +213,14 @@ template <class R> struct Engine3D {
     }
     // planes [3, 7) surface band, [7, flat) terrain band, [max(7, flat), nz - 3) flat band
     int flatFrom() const { return std::clamp(m.flat, std::min(7, nz - 3), nz - 3); }
+    // VGPRs of the flat band's kernel as configured
+    int flatRegisters() const {
+        const void *k = vectorFlat == 4 ? reinterpret_cast<const void *>(&band_vec_kernel<R, 0, 0, 0, 16 / sizeof(R), FLAT>)
+                        : vectorFlat == 2 ? reinterpret_cast<const void *>(&band_vec_kernel<R, 0, 0, 0, 2, FLAT>)
+                                          : reinterpret_cast<const void *>(&stage3_kernel<R, 0, 0, 0, FLAT>);
+        hipFuncAttributes attr{};
+        return hipFuncGetAttributes(&attr, k) == hipSuccess ? attr.numRegs : 0;
+    }
     // tiles per plane of a deep band: TX x TY nodes, or with nv nodes per lane 128-byte row segments by
     // THREADS / (128 / (nv * sizeof(R))) rows
     void bandTiles(int nv, int &tx, int &ty) const {
@@ -576,7 +581,7 @@ int swl_engine3d_kernel_timing(swl_engine3d *e, double *out) {
         e->with([&](auto &x) {
             out[0] = x.stageMs, out[1] = (double)x.stageLaunches;
             out[2] = (double)(x.nx - 6) * (double)(x.ny - 6) * (double)(x.nz - 6);
-            out[3] = x.registers, out[4] = x.m.flat, out[5] = x.m.by_depth;
+            out[3] = x.flatRegisters(), out[4] = x.m.flat, out[5] = x.m.by_depth;
         });
     });
 }
