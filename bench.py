@@ -255,16 +255,7 @@ def measure_batch(pkg, shots, steps):
     p32 = workloads.gsls_batch(shots=shots, precision=pkg.Precision.Float32, steps=steps)
     p64 = workloads.gsls_batch(shots=shots, precision=pkg.Precision.Float64, steps=steps)
     t0 = time.perf_counter()
-    sims = []
-    for p in p32:
-        s = pkg.Simulation()
-        ok, err = s.initialize(p, pkg.BackendKind.Hip, 0)
-        if not ok:
-            raise RuntimeError(err)
-        sims.append(s)
-    batch = pkg.Batch2D(0)
-    for s in sims:
-        batch.add(s)
+    batch, sims = pkg.Batch2D.build(p32, 0, workers=8)  # sessions built on 8 host threads
     t1 = time.perf_counter()
     ok, err = batch.advance(steps)
     t2 = time.perf_counter()

@@ -746,3 +746,29 @@ class Batch2D:
         o = np.zeros(len(self.STAT_NAMES))
         native.lib().swl_batch2d_stats(self._h, _dp(o))
         return dict(zip(self.STAT_NAMES, o.tolist()))
+
+    @staticmethod
+    def build(params, device: int = 0, workers: int = 8):
+        """Creates one HIP session per Parameters on `workers` threads (the native session builds run outside
+        the GIL and overlap each other's host work and uploads) and a batch of them in the given order.
+        Returns (batch, sessions); on a failed build the sessions already made are closed and RuntimeError names
+        the shot."""
+        import concurrent.futures as cf
+
+        def one(p):
+            s = Simulation()
+            ok, err = s.initialize(p, BackendKind.Hip, device)
+            return s, ("" if ok else err)
+
+        with cf.ThreadPoolExecutor(max_workers=max(1, int(workers))) as pool:
+            built = list(pool.map(one, params))
+        failed = [(i, e) for i, (_, e) in enumerate(built) if e]
+        if failed:
+            for s, _ in built:
+                s.close()
+            raise RuntimeError(f"shot {failed[0][0]}: {failed[0][1]}")
+        batch = Batch2D(device)
+        sims = [s for s, _ in built]
+        for s in sims:
+            batch.add(s)
+        return batch, sims

@@ -87,6 +87,25 @@ def test_batch_equals_alone(mode):
         assert st["direct_chunks"] == st["chunks"]
 
 
+def test_parallel_build_equals_alone():
+    """Batch2D.build creates the sessions on several host threads: every shot is still bit-identical to running
+    it alone, in the order given."""
+    ps = shots()
+    ref = [alone(copy.deepcopy(p)) for p in ps]
+    batch, members = Batch2D.build([copy.deepcopy(p) for p in ps], 0, workers=4)
+    assert len(batch) == 4 and len(members) == 4
+    ok, err = batch.advance(1000)
+    assert ok, err
+    for s, r in zip(members, ref):
+        assert s.completed()
+        same(s, r)
+    batch.close()
+    bad = copy.deepcopy(ps[0])
+    bad.nx = 3  # rejected by the session's validation
+    with pytest.raises(RuntimeError, match="shot 1"):
+        Batch2D.build([copy.deepcopy(ps[1]), bad], 0, workers=2)
+
+
 def test_lanes_uneven_calls_equal_alone():
     """Lanes with members finishing at different steps and calls that split runs unevenly."""
     ps = shots()
