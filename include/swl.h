@@ -193,6 +193,16 @@ typedef struct swl_run2d {
 } swl_run2d;
 int swl_engine2d_run(swl_engine2d *e, int first_step, int count, double dt, double frequency, double scale,
                      double peak_energy, double *diag, double *samples, swl_run2d *result);
+/* Pipelined form of swl_engine2d_run (no reference counterpart; the same steps as the reference's advance()
+   loop, CurvilinearSolver2D.cpp:553-597): swl_engine2d_run_enqueue returns once `count` steps are enqueued
+   with their rows in `slot` (0 or 1); swl_engine2d_run_collect waits for that run and copies its rows, as
+   swl_engine2d_run returns them.  A caller enqueues the next run before it collects this one, so its host work
+   on the rows overlaps the device's steps.  fresh = 1 starts from `peak_energy` with the fault latch cleared;
+   fresh = 0 continues the run before it (same slot pairing): the device's peak and fault latch carry over, so a
+   run enqueued after a faulted one accepts no step.  A slot is collected once per enqueue. */
+int swl_engine2d_run_enqueue(swl_engine2d *e, int slot, int first_step, int count, double dt, double frequency,
+                             double scale, double peak_energy, int fresh);
+int swl_engine2d_run_collect(swl_engine2d *e, int slot, double *diag, double *samples, swl_run2d *result);
 /* Timing with HIP events on the engine's stream during swl_engine2d_run (no reference
    counterpart; the reference times whole steps on the host).  One event pair brackets the four
    back-to-back stage launches of a step; the finalisation is timed on every 32nd step.

@@ -11,23 +11,6 @@
 namespace swl {
 namespace {
 
-struct PinnedBuffer { // page-locked host staging (hipHostMalloc; never registers existing memory)
-    void *ptr = nullptr;
-    size_t bytes = 0;
-    PinnedBuffer() = default;
-    PinnedBuffer(const PinnedBuffer &) = delete;
-    PinnedBuffer &operator=(const PinnedBuffer &) = delete;
-    ~PinnedBuffer() {
-        if (ptr) (void)hipHostFree(ptr);
-    }
-    void reserve(size_t n) {
-        if (n <= bytes) return;
-        if (ptr) (void)hipHostFree(ptr), ptr = nullptr;
-        check(hipHostMalloc(&ptr, n, hipHostMallocDefault), "pinned staging allocation");
-        bytes = n;
-    }
-};
-
 struct BatchLaunch {
     int kind; // 0 stage, 1 final, 2 begin
     int group, dx, dz, mech, grid;
@@ -215,7 +198,7 @@ struct Runner2D {
                                 a.kind = -1;
                                 if (j < std::min(left[idx], chunk) &&
                                     Engine2D<R>::direction(step, s, 0) == dx && Engine2D<R>::direction(step, s, 2) == dz) {
-                                    const int c = (e.cur + 2 * j) % 3;
+                                    const int c = (e.cur + 2 * j) % Engine2D<R>::NB;
                                     a = e.stage_args(c, S[idx]->dt, S[idx]->frequency, S[idx]->scale, step, s);
                                     any = true;
                                 }
@@ -241,7 +224,7 @@ struct Runner2D {
                             f.status = e.status.template as<int>(), f.accepted = e.accepted.template as<int>();
                             f.peak = e.peak.template as<double>();
                             f.diag = e.diag.template as<double>(), f.samples = e.samples.template as<double>();
-                            f.state = e.S((e.cur + 2 * (j + 1)) % 3), f.probes = e.probes.template as<long long>();
+                            f.state = e.S((e.cur + 2 * (j + 1)) % Engine2D<R>::NB), f.probes = e.probes.template as<long long>();
                             f.nrec = e.nrec, f.row = j, f.step = S[idx]->first_step + done[idx] + j;
                             f.dt = S[idx]->dt, f.frequency = S[idx]->frequency;
                         }
@@ -367,8 +350,8 @@ struct Runner2D {
                         check(hipMemcpy(S[i]->samples + 2 * (size_t)e.nrec * done[i], e.samples.ptr,
                                         (size_t)a * 2 * e.nrec * sizeof(double), hipMemcpyDeviceToHost),
                               "batch samples");
-                    e.prev = (e.cur + 2 * (a - 1)) % 3;
-                    e.cur = (e.cur + 2 * a) % 3;
+                    e.prev = (e.cur + 2 * (a - 1)) % Engine2D<R>::NB;
+                    e.cur = (e.cur + 2 * a) % Engine2D<R>::NB;
                 }
                 done[i] += a, peak[i] = pk[i];
                 S[i]->result.accepted = done[i], S[i]->result.peak_energy = pk[i];

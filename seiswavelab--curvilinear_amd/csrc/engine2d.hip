@@ -6,7 +6,7 @@
 // runs reproduce the reference's CPU path to rounding.  The execution model is the device's own:
 //   * one fused kernel pass per RK stage: RHS + stage combine + surface projection + PML / GSLS
 //     memory update; the stage-4 pass also produces the energy / finiteness inspection;
-//   * three rotating state buffers plus one accumulator (acc = k1 + 2k2 + 2k3, rounded exactly as the
+//   * four rotating state buffers plus one accumulator (acc = k1 + 2k2 + 2k3, rounded exactly as the
 //     reference's a+2b+2c+d) instead of seven RK field sets; rollback is a buffer-index change;
 //   * an aligned padded layout (node (i,k) at off + i + ld*k, column 3 on a 256-byte boundary, ld a
 //     multiple of 64) so the bulk of the grid streams with 16-byte lane accesses;
@@ -20,7 +20,8 @@
 //   * tiles mapped XCD-contiguously (each of the 8 XCDs sweeps its own band of rows so the stencil's
 //     z-neighbours are reused in that XCD's L2);
 //   * device-side fault rule, peak tracking and receiver gather (swl_engine2d_run): no per-step
-//     host synchronisation.
+//     host synchronisation; each step is ended by one block of the next step's first stage launch, so a
+//     step is four launches.
 #include "common.hpp"
 #include "swl.h"
 #include <algorithm>
@@ -77,6 +78,23 @@ struct Partial {
     int bad, pad;
 };
 
+// End of the previous step in a device-resident run, done by block 0 of the next step's first stage launch
+// while the other blocks work: the reduction of the previous step's inspection partials (written by the
+// previous launch), the fault rule and, if the step is accepted, its receiver row read from the stage input
+// (which is that step's state).  The engine keeps four rotating state buffers, so the first stage writes
+// neither the accepted state nor the one before it: a step rejected here still leaves the state it started
+// from intact, and every later launch sees the fault and does nothing.
+struct StepEnd {
+    const Partial *partials; // the previous step's partials
+    int n;
+    int *status, *accepted;
+    double *peak, *diag, *samples;
+    const long long *probes;
+    int nrec;
+    int row, step; // row and step number of the previous step
+    double dt, frequency;
+};
+
 template <class R> struct StageArgs {
     DMesh<R> m;
     const R *in, *base;
@@ -94,7 +112,8 @@ template <class R> struct StageArgs {
     const int2 *border;      // (i, k) of every node outside the interior rectangle, row by row, 256 per block
     int nborder;             // border nodes
     int ngeneral, gblocks, ninterior, rtx;
-    int nt; // interior: nontemporal stores
+    int nt;          // interior: nontemporal stores
+    StepEnd end;     // first stage of a device-resident run: end.partials non-null ends the previous step
 };
 
 // ---- tile mapping: contiguous band of tiles per XCD (blocks are dispatched round-robin over 8 XCDs)
@@ -439,7 +458,6 @@ __device__ void block_reduce(Partial &p, Partial *sh) {
         __syncthreads();
     }
 }
-
 // energy (PointKernels2D.hpp:181-184) and inspection terms (CurvilinearSolver2D.cpp:102-111) of one node
 // FAST: one FP64 division (xz^2/(2mu) = xz^2 * 4(lambda+mu) / (8 mu (lambda+mu))), same value to rounding
 template <class R, bool FAST = false>
@@ -1023,12 +1041,57 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
 // take the interior tiles.  A single in-order launch avoids cross-stream joins between stages.
 // The FP32 elastic stage fits 96 VGPRs without spilling, which gives 5 waves per SIMD instead of 4.
 template <class R, int MECH> constexpr int kWavesPerEU = sizeof(R) == 4 && MECH == 0 ? 5 : 1;
+
+// The reference's fault rule and peak update for step `step` on the reduced inspection r
+// (CurvilinearSolver2D.cpp:576-590): not finite, or after 3/f an energy above 1e4 x the accepted peak, faults the
+// step; otherwise the peak, the diagnostics row and the accepted count advance.  One thread.  Returns the fault.
+__device__ inline bool apply_rule(const Partial &r, int *status, int *accepted, double *peak, double *diag, int row,
+                                  int step, double dt, double frequency) {
+    const bool finite = !r.bad && isfinite(r.energy);
+    const double before = *peak;
+    const bool fault = !finite || ((step + 1) * dt > 3 / frequency && before > 0 && r.energy > before * 10000);
+    if (fault) {
+        *status = 1;
+    } else {
+        *peak = fmax(before, r.energy);
+        double *d = diag + 4 * (size_t)row;
+        d[0] = r.energy, d[1] = r.velocity, d[2] = r.traction, d[3] = 1;
+        *accepted = row + 1;
+    }
+    return fault;
+}
+// receiver row `row`: Vx and Vz at every probe of the state (whole block)
+template <class R>
+__device__ inline void sample_state(const R *state, const long long *probes, int nrec, double *samples, int row) {
+    for (int r = threadIdx.x; r < nrec; r += blockDim.x) {
+        double *o = samples + (size_t)row * 2 * nrec + 2 * r;
+        o[0] = state[probes[2 * r]];
+        o[1] = state[probes[2 * r + 1]];
+    }
+}
+// The previous step's end (see StepEnd), by one block of a first stage launch.
+template <class R> __device__ void end_previous_step(const StageArgs<R> &A, Partial *sh) {
+    __shared__ int verdict;
+    const StepEnd &E = A.end;
+    Partial p{0, 0, 0, 0, 0};
+    for (int t = threadIdx.x; t < E.n; t += THREADS) merge(p, E.partials[t]);
+    block_reduce(p, sh);
+    if (threadIdx.x == 0)
+        verdict = apply_rule(sh[0], E.status, E.accepted, E.peak, E.diag, E.row, E.step, E.dt, E.frequency);
+    __syncthreads();
+    if (!verdict) sample_state(A.in, E.probes, E.nrec, E.samples, E.row);
+}
+
 template <class R, int DX, int DZ, int MECH, bool TILE = false>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<R, MECH>))) void stage_kernel(
     StageArgs<R> A) {
     __shared__ Partial sh[THREADS];
     if (*A.status) return;
     const int b = blockIdx.x;
+    if (A.end.partials && b == 0) { // block 0 is dispatched first: the previous step's end overlaps the launch
+        end_previous_step(A, sh);
+        __syncthreads(); // sh is used again below
+    }
     if (b < A.gblocks) {
         if (b < A.ngeneral) general_body<R, DX, DZ, MECH>(A, b, sh);
     } else if constexpr (TILE) {
@@ -1110,6 +1173,10 @@ template <class R> __global__ __launch_bounds__(THREADS) void batch_final_kernel
 __global__ void run_begin_kernel(int *status, int *accepted, double *peak, double peak0) {
     if (threadIdx.x == 0) *status = 0, *accepted = 0, *peak = peak0;
 }
+// start of a continuation run: its rows count from 0; the fault latch and the accepted peak carry over
+__global__ void run_continue_kernel(int *accepted) {
+    if (threadIdx.x == 0) *accepted = 0;
+}
 __global__ void batch_begin_kernel(int *const *status, int *const *accepted, double *const *peak, const double *peak0,
                                    int n) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1123,28 +1190,11 @@ template <class R> __device__ void finalize_body(const FinalArgs<R> &A) {
     Partial p{0, 0, 0, 0, 0};
     for (int t = threadIdx.x; t < A.n; t += THREADS) merge(p, A.partials[t]);
     block_reduce(p, sh);
-    if (threadIdx.x == 0) {
-        const Partial r = sh[0];
-        const bool finite = !r.bad && isfinite(r.energy);
-        const double before = *A.peak;
-        const bool fault = !finite || ((A.step + 1) * A.dt > 3 / A.frequency && before > 0 && r.energy > before * 10000);
-        if (fault) {
-            *A.status = 1;
-        } else {
-            *A.peak = fmax(before, r.energy);
-            double *d = A.diag + 4 * (size_t)A.row;
-            d[0] = r.energy, d[1] = r.velocity, d[2] = r.traction, d[3] = 1;
-            *A.accepted = A.row + 1;
-        }
-        verdict = fault;
-    }
+    if (threadIdx.x == 0)
+        verdict = apply_rule(sh[0], A.status, A.accepted, A.peak, A.diag, A.row, A.step, A.dt, A.frequency);
     __syncthreads();
     if (verdict) return;
-    for (int r = threadIdx.x; r < A.nrec; r += THREADS) {
-        double *o = A.samples + (size_t)A.row * 2 * A.nrec + 2 * r;
-        o[0] = A.state[A.probes[2 * r]];
-        o[1] = A.state[A.probes[2 * r + 1]];
-    }
+    sample_state(A.state, A.probes, A.nrec, A.samples, A.row);
 }
 
 // Inspection of the accepted state (CudaEngine::diagnostics path), 64x4 tiles over all nodes.

@@ -36,6 +36,23 @@ ScratchSlot &scratch_slot(int device) {
     return *slots[device];
 }
 
+struct PinnedBuffer { // page-locked host staging (hipHostMalloc; never registers existing memory)
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    PinnedBuffer() = default;
+    PinnedBuffer(const PinnedBuffer &) = delete;
+    PinnedBuffer &operator=(const PinnedBuffer &) = delete;
+    ~PinnedBuffer() {
+        if (ptr) (void)hipHostFree(ptr);
+    }
+    void reserve(size_t n) {
+        if (n <= bytes) return;
+        if (ptr) (void)hipHostFree(ptr), ptr = nullptr;
+        check(hipHostMalloc(&ptr, n, hipHostMallocDefault), "pinned staging allocation");
+        bytes = n;
+    }
+};
+
 inline uint64_t next_engine_uid() {
     static std::atomic<uint64_t> counter{0};
     return ++counter;
@@ -51,13 +68,17 @@ template <class R> struct Engine2D {
     hipStream_t stream{}, side{};
     DMesh<R> m{};
     DeviceBuffer mesh_arrays[32];
-    DeviceBuffer state[3], accbuf, pstate[3], paccbuf, partials, level2, probes, status, accepted, peak, diag, samples,
+    // NB rotating state buffers (and PML memories): a step from buffer c writes c + 1 and c + 2 (mod NB); with four,
+    // the next step's first stage writes neither c + 2 nor c, so that stage can end the step before it (StepEnd)
+    static constexpr int NB = 4;
+    DeviceBuffer state[NB], accbuf, pstate[NB], paccbuf, partials, level2, probes, status, accepted, peak, diag, samples,
         gather, display, borderNodes;
     int ntx = 0, ntiles = 0;      // 64x4 tiles over the computational grid
     int nBorder = 0, nGeneral = 0; // nodes outside the interior rectangle, blocks of 256 of them
     int rtx = 0, nInterior = 0;   // interior tiles (64*NV x 4 nodes)
     int nPartials = 0, nLevel2 = 0;
     static constexpr int kDirectPartials = 2048;
+    int endInStage = 1; // device-resident runs end each step in the next step's first stage launch
     hipEvent_t fork[4]{}, join[4]{};
     int cur = 0, prev = 0, diag_rows = 0, registers = 0, nt = 1, metricOk = 0;
     int tileLds = 0; // elastic interior: stencil input staged through LDS tiles
@@ -217,6 +238,9 @@ template <class R> struct Engine2D {
         }
     }
     ~Engine2D() {
+        if (stream) (void)hipStreamSynchronize(stream); // a pipelined run may still be in flight
+        for (auto &P : pending)
+            if (P.done) (void)hipEventDestroy(P.done);
         for (auto e : events) (void)hipEventDestroy(e);
         for (int e = 0; e < 4; ++e) {
             if (fork[e]) (void)hipEventDestroy(fork[e]);
@@ -373,10 +397,10 @@ template <class R> struct Engine2D {
         return (1 - 2 * a * a) * std::exp(-a * a);
     }
 
-    // Arguments of RK stage s (0..3) of step `step` taken from buffer c (three rotating buffers:
-    // stages read c, b1, b2, b1 and write b1, b2, b1, b2; the step ends in b2 = (c + 2) % 3).
+    // Arguments of RK stage s (0..3) of step `step` taken from buffer c (stages read c, b1, b2, b1 and write
+    // b1, b2, b1, b2; the step ends in b2 = (c + 2) % NB).
     StageArgs<R> stage_args(int c, double dt, double frequency, double scale, int step, int s) const {
-        const int b1 = (c + 1) % 3, b2 = (c + 2) % 3;
+        const int b1 = (c + 1) % NB, b2 = (c + 2) % NB;
         const int in[4] = {c, b1, b2, b1}, out[4] = {b1, b2, b1, b2};
         const double times[4] = {step * dt, (step + .5) * dt, (step + .5) * dt, (step + 1) * dt};
         const R weights[4] = {(R)(dt / 2), (R)(dt / 2), (R)dt, (R)dt};
@@ -401,7 +425,7 @@ template <class R> struct Engine2D {
     int enqueue_step(int c, double dt, double frequency, double scale, int step) {
         for (int s = 0; s < 4; ++s)
             launch_stage(direction(step, s, 0), direction(step, s, 2), stage_args(c, dt, frequency, scale, step, s), s);
-        return (c + 2) % 3;
+        return (c + 2) % NB;
     }
 
     void step(double, double dt, double frequency, double scale, int number) {
@@ -549,36 +573,65 @@ template <class R> struct Engine2D {
         samples.allocate((size_t)count_steps * 2 * std::max(nrec, 1) * sizeof(double));
         diag_rows = count_steps;
     }
+    // rows the enqueued kernels write: slot 0 diag / samples, slot 1 diagAlt / samplesAlt (pipelined runs)
+    DeviceBuffer diagAlt, samplesAlt;
+    int rowSlot = 0, altRows = 0;
+    double *rows_diag() const { return (rowSlot ? diagAlt : diag).as<double>(); }
+    double *rows_samples() const { return (rowSlot ? samplesAlt : samples).as<double>(); }
     void run_begin(hipStream_t s, double peak0) {
         run_begin_kernel<<<1, 64, 0, s>>>(status.as<int>(), accepted.as<int>(), peak.as<double>(), peak0);
         check(hipGetLastError(), "2D run start");
     }
     void run_steps(hipStream_t s, int first, int j0, int j1, double dt, double frequency, double scale) {
-        require(j1 <= diag_rows, "run rows not reserved");
+        require(j1 <= (rowSlot ? altRows : diag_rows), "run rows not reserved");
         OnStream on(stream, s);
-        int c = (cur + 2 * j0) % 3;
+        int c = (cur + 2 * j0) % NB;
+        if (endInStage) {
+            // step j ends in the first stage launch of step j + 1; the run's last step by the kernels below
+            for (int j = j0; j < j1; ++j) {
+                const int st = first + j;
+                for (int q = 0; q < 4; ++q) {
+                    StageArgs<R> a = stage_args(c, dt, frequency, scale, st, q);
+                    if (q == 0 && j > j0) {
+                        StepEnd &E = a.end;
+                        E.partials = partials.as<Partial>(), E.n = nPartials;
+                        E.status = status.as<int>(), E.accepted = accepted.as<int>(), E.peak = peak.as<double>();
+                        E.diag = rows_diag(), E.samples = rows_samples(), E.probes = probes.as<long long>();
+                        E.nrec = nrec, E.row = j - 1, E.step = st - 1, E.dt = dt, E.frequency = frequency;
+                    }
+                    launch_stage(direction(st, q, 0), direction(st, q, 2), a, q);
+                }
+                c = (c + 2) % NB;
+            }
+            if (j1 > j0) finish_step(j1 - 1, first + j1 - 1, c, dt, frequency);
+            return;
+        }
         for (int j = j0; j < j1; ++j) {
             const int st = first + j;
             const int n = enqueue_step(c, dt, frequency, scale, st);
             const bool timed = j % 32 == 0; // finalisation is timed on every 32nd step only (events add gaps)
             if (timed) mark(1, true);
-            // up to kDirectPartials tile partials the finalisation block reduces them itself (one launch
-            // fewer per step); beyond, a first level spreads the reduction over up to 128 blocks
-            const bool direct = nPartials <= kDirectPartials;
-            if (!direct)
-                reduce_kernel<<<nLevel2, THREADS, 0, stream>>>(partials.as<Partial>(), nPartials, level2.as<Partial>(),
-                                                              status.as<int>());
-            FinalArgs<R> f{};
-            f.partials = direct ? partials.as<Partial>() : level2.as<Partial>(), f.n = direct ? nPartials : nLevel2;
-            f.status = status.as<int>(), f.accepted = accepted.as<int>(), f.peak = peak.as<double>();
-            f.diag = diag.as<double>(), f.samples = samples.as<double>();
-            f.state = S(n), f.probes = probes.as<long long>(), f.nrec = nrec, f.row = j, f.step = st;
-            f.dt = dt, f.frequency = frequency;
-            finalize_kernel<R><<<1, THREADS, 0, stream>>>(f);
+            finish_step(j, st, n, dt, frequency);
             if (timed) mark(1, false);
-            check(hipGetLastError(), "2D finalize launch");
             c = n;
         }
+    }
+    // Separate kernels end step `st` (diagnostics row j, new state in buffer n): up to kDirectPartials tile
+    // partials the finalisation block reduces them itself; beyond, a first level spreads the reduction over up to
+    // 128 blocks.
+    void finish_step(int j, int st, int n, double dt, double frequency) {
+        const bool direct = nPartials <= kDirectPartials;
+        if (!direct)
+            reduce_kernel<<<nLevel2, THREADS, 0, stream>>>(partials.as<Partial>(), nPartials, level2.as<Partial>(),
+                                                          status.as<int>());
+        FinalArgs<R> f{};
+        f.partials = direct ? partials.as<Partial>() : level2.as<Partial>(), f.n = direct ? nPartials : nLevel2;
+        f.status = status.as<int>(), f.accepted = accepted.as<int>(), f.peak = peak.as<double>();
+        f.diag = rows_diag(), f.samples = rows_samples();
+        f.state = S(n), f.probes = probes.as<long long>(), f.nrec = nrec, f.row = j, f.step = st;
+        f.dt = dt, f.frequency = frequency;
+        finalize_kernel<R><<<1, THREADS, 0, stream>>>(f);
+        check(hipGetLastError(), "2D finalize launch");
     }
     void run_end(hipStream_t s, Verdict *v) {
         check(hipMemcpyAsync(&v->accepted, accepted.ptr, sizeof(int), hipMemcpyDeviceToHost, s), "2D run readback");
@@ -599,9 +652,9 @@ template <class R> struct Engine2D {
         res.accepted = a;
         res.faulted = v.status != 0;
         res.peak_energy = v.peak;
-        // the step from buffer c ends in (c + 2) % 3
-        prev = a ? (cur + 2 * (a - 1)) % 3 : cur;
-        cur = (cur + 2 * a) % 3;
+        // the step from buffer c ends in (c + 2) % NB
+        prev = a ? (cur + 2 * (a - 1)) % NB : cur;
+        cur = (cur + 2 * a) % NB;
     }
     void run(int first, int count_steps, double dt, double frequency, double scale, double peak0, double *diag_out,
              double *samples_out, swl_run2d &res) {
@@ -616,6 +669,82 @@ template <class R> struct Engine2D {
         run_end(stream, &v);
         check(hipStreamSynchronize(stream), "2D run");
         run_finish(v, diag_out, samples_out, res);
+    }
+
+    // ---- pipelined runs: a caller enqueues the next run before it reads this one's rows, so the host's work
+    // on them overlaps the device's.  Two row slots; a continuation run keeps the device's fault latch and
+    // accepted peak (the device has already applied the previous run's steps to them).  Each slot remembers
+    // the buffer its run starts from: the state buffer advances by the steps actually accepted, and a run
+    // enqueued after a faulted one accepts none.
+    struct Pending {
+        int start = 0, count = 0, active = 0;
+        hipEvent_t done{};
+    };
+    Pending pending[2];
+    PinnedBuffer verdictHost; // one Verdict per slot
+    int plannedCur = 0;       // buffer after every enqueued run, if all their steps are accepted
+    void run_enqueue(int slot, int first, int count_steps, double dt, double frequency, double scale, double peak0,
+                     bool fresh) {
+        DeviceScope scope(device);
+        require(slot == 0 || slot == 1, "run slot must be 0 or 1");
+        require(count_steps > 0, "a pipelined run needs at least one step");
+        Pending &P = pending[slot];
+        require(!P.active, "run slot already in use");
+        require(fresh || pending[slot ^ 1].active || plannedCur == cur, "continuation without a run before it");
+        if (slot) {
+            if (altRows < count_steps) {
+                diagAlt.allocate((size_t)count_steps * 4 * sizeof(double));
+                samplesAlt.allocate((size_t)count_steps * 2 * std::max(nrec, 1) * sizeof(double));
+                altRows = count_steps;
+            }
+        } else {
+            reserve_rows(count_steps);
+        }
+        verdictHost.reserve(2 * sizeof(Verdict));
+        if (!P.done) check(hipEventCreateWithFlags(&P.done, hipEventDisableTiming), "event");
+        if (fresh) {
+            plannedCur = cur;
+            run_begin(stream, peak0);
+        } else {
+            run_continue_kernel<<<1, 64, 0, stream>>>(accepted.as<int>());
+            check(hipGetLastError(), "2D run continuation");
+        }
+        const int keep = cur;
+        cur = plannedCur, rowSlot = slot;
+        run_steps(stream, first, 0, count_steps, dt, frequency, scale);
+        cur = keep, rowSlot = 0;
+        run_end(stream, static_cast<Verdict *>(verdictHost.ptr) + slot);
+        check(hipEventRecord(P.done, stream), "2D run event");
+        P.start = plannedCur, P.count = count_steps, P.active = 1;
+        plannedCur = (plannedCur + 2 * count_steps) % NB;
+    }
+    void run_collect(int slot, double *diag_out, double *samples_out, swl_run2d &res) {
+        DeviceScope scope(device);
+        require(slot == 0 || slot == 1, "run slot must be 0 or 1");
+        Pending &P = pending[slot];
+        require(P.active, "no run enqueued in this slot");
+        check(hipEventSynchronize(P.done), "2D run");
+        P.active = 0;
+        const Verdict v = static_cast<const Verdict *>(verdictHost.ptr)[slot];
+        const int a = v.accepted;
+        if (a) {
+            if (diag_out)
+                check(hipMemcpy(diag_out, (slot ? diagAlt : diag).ptr, (size_t)a * 4 * sizeof(double), hipMemcpyDeviceToHost),
+                      "2D run diagnostics");
+            if (samples_out && nrec)
+                check(hipMemcpy(samples_out, (slot ? samplesAlt : samples).ptr, (size_t)a * 2 * nrec * sizeof(double),
+                                hipMemcpyDeviceToHost),
+                      "2D run samples");
+            prev = (P.start + 2 * (a - 1)) % NB;
+            cur = (P.start + 2 * a) % NB;
+        }
+        res.accepted = a;
+        res.faulted = v.status != 0;
+        res.peak_energy = v.peak;
+        if (!pending[slot ^ 1].active) {
+            plannedCur = cur;
+            collect(); // profiling events, once nothing is in flight
+        }
     }
 };
 
@@ -711,9 +840,19 @@ int swl_engine2d_set_option(swl_engine2d *e, const char *name, int value) {
             else if (key == "stored_metrics") x.m.metric = value == 0 && x.metricOk; // read a, b, J from memory
             else if (key == "lds_tiles") x.tileLds = value != 0;
             else if (key == "material_rows") x.m.by_row = value != 0 && x.rowsOk; // 0: read the per-node arrays
+            else if (key == "end_in_stage") x.endInStage = value != 0; // 0: reduce + finalize kernels after every step
             else throw std::runtime_error("unknown engine option: " + key);
         });
     });
+}
+int swl_engine2d_run_enqueue(swl_engine2d *e, int slot, int first_step, int count, double dt, double frequency,
+                             double scale, double peak_energy, int fresh) {
+    return swl::guarded([&] {
+        e->with([&](auto &x) { x.run_enqueue(slot, first_step, count, dt, frequency, scale, peak_energy, fresh != 0); });
+    });
+}
+int swl_engine2d_run_collect(swl_engine2d *e, int slot, double *diag, double *samples, swl_run2d *result) {
+    return swl::guarded([&] { e->with([&](auto &x) { x.run_collect(slot, diag, samples, *result); }); });
 }
 int swl_engine2d_run(swl_engine2d *e, int first_step, int count, double dt, double frequency, double scale,
                      double peak_energy, double *diag, double *samples, swl_run2d *result) {

@@ -280,10 +280,10 @@ double hipDeviceBytes2D(const swl_params2d &p, int rows) {
     const double cpml = cpmlScalars2D(p), nrec = std::max(p.receiver_count, 1);
     const double tiles = std::ceil(d[0] / 64.0) * std::ceil(d[1] / 4.0);
     if (p.max_steps > 0) rows = std::min(rows, p.max_steps); // a run never reserves more rows than it has steps
-    double bytes = 4.0 * fields * count * word;                                                 // 3 rotating states + accumulator
+    double bytes = 5.0 * fields * count * word;                                                 // 4 rotating states + accumulator
     bytes += count * word * (7 + 2 + (sponge ? 1 : 0) + (explosive ? 2 : 0) + 2 * mech);     // mesh arrays (normals: 2)
     bytes += 8.0 * (6 * nx + 2 * nx + 2 * nz);                                                // FP64 surface row, terrain profiles
-    if (cpml) bytes += 4.0 * cpml * word + 3.0 * (d[0] + d[1]) * word * (word == 4 ? 2 : 1);  // memories, profiles
+    if (cpml) bytes += 5.0 * cpml * word + 3.0 * (d[0] + d[1]) * word * (word == 4 ? 2 : 1);  // memories, profiles
     bytes += (3.0 + 2 * mech) * nz * word;                                                     // per-row material tables
     bytes += 8.0 * nodes;                                                                      // border-node list (bound)
     bytes += 32.0 * (tiles + std::ceil(nodes / 256) + 1) + 32.0 * 128;                         // inspection partials

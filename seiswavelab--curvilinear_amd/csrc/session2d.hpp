@@ -69,24 +69,46 @@ struct swl_sim2d {
             throw std::runtime_error(fault);
         }
     }
+    // Runs of up to `chunk` steps, pipelined: run k + 1 is enqueued (continuing on the device from run k) before
+    // run k's rows are read and folded, so the host's receiver resampling overlaps the device's steps.
+    std::vector<double> diagRowsAlt, sampleRowsAlt;
     void advance(int count) {
         if (!fault.empty()) throw std::runtime_error(fault);
         constexpr int chunk = 512;
         const size_t nrec = model.receiverX.size();
-        while (count > 0 && step < model.total) {
-            const int n = std::min({count, model.total - step, chunk});
-            diagRows.resize(4 * static_cast<size_t>(n));
-            sampleRows.resize(2 * nrec * static_cast<size_t>(n));
+        std::vector<int> runs;
+        for (int left = std::min(count, model.total - step); left > 0; left -= chunk) runs.push_back(std::min(left, chunk));
+        if (runs.empty()) return;
+        const auto failed = [&] {
+            fault = "Compute backend failed: " + swl::last_error() + ". Rebuild the session.";
+            return std::runtime_error(fault);
+        };
+        const auto enqueue = [&](size_t k, int first) {
+            const int slot = static_cast<int>(k & 1);
+            if (swl_engine2d_run_enqueue(engine, slot, first, runs[k], model.dt, model.p.frequency, scale(), peak, k == 0))
+                throw failed();
+        };
+        for (auto *rows : {&diagRows, &diagRowsAlt}) rows->resize(4 * static_cast<size_t>(runs[0]));
+        for (auto *rows : {&sampleRows, &sampleRowsAlt}) rows->resize(2 * nrec * static_cast<size_t>(runs[0]));
+        enqueue(0, step);
+        auto start = std::chrono::steady_clock::now();
+        for (size_t k = 0; k < runs.size(); ++k) {
+            const int slot = static_cast<int>(k & 1);
+            if (k + 1 < runs.size()) enqueue(k + 1, step + runs[k]);
             swl_run2d res{};
-            const auto start = std::chrono::steady_clock::now();
-            if (swl_engine2d_run(engine, step, n, model.dt, model.p.frequency, scale(), peak, diagRows.data(),
-                                 sampleRows.data(), &res)) {
-                fault = "Compute backend failed: " + swl::last_error() + ". Rebuild the session.";
-                throw std::runtime_error(fault);
+            double *d = slot ? diagRowsAlt.data() : diagRows.data(), *s = slot ? sampleRowsAlt.data() : sampleRows.data();
+            if (swl_engine2d_run_collect(engine, slot, d, s, &res)) throw failed();
+            if (res.accepted < runs[k] && k + 1 < runs.size()) { // the run after a faulted one accepts nothing
+                swl_run2d rest{};
+                if (swl_engine2d_run_collect(engine, slot ^ 1, nullptr, nullptr, &rest)) throw failed();
             }
-            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count();
+            const auto now = std::chrono::steady_clock::now();
+            const double ms = std::chrono::duration<double, std::milli>(now - start).count();
+            start = now;
+            if (slot) diagRows.swap(diagRowsAlt), sampleRows.swap(sampleRowsAlt); // absorb() reads diagRows / sampleRows
             absorb(res, ms);
-            count -= n;
+            if (slot) diagRows.swap(diagRowsAlt), sampleRows.swap(sampleRowsAlt);
+            if (res.accepted < runs[k]) break;
         }
     }
 };

@@ -48,6 +48,9 @@ SIGNATURES = {
     "swl_engine2d_resident_bytes": (C.c_uint64, [vp]),
     "swl_engine2d_run": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, vp, vp,
                                    P(Run2D)]),
+    "swl_engine2d_run_enqueue": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                           C.c_double, C.c_int]),
+    "swl_engine2d_run_collect": (C.c_int, [vp, C.c_int, vp, vp, P(Run2D)]),
     "swl_engine2d_profile": (C.c_int, [vp, C.c_int]),
     "swl_engine2d_kernel_timing": (C.c_int, [vp, vp]),
     "swl_engine2d_set_option": (C.c_int, [vp, C.c_char_p, C.c_int]),
