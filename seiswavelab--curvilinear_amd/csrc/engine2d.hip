@@ -113,6 +113,7 @@ template <class R> struct StageArgs {
     int nborder;             // border nodes
     int ngeneral, gblocks, ninterior, rtx;
     int nt;          // interior: nontemporal stores
+    int spread;      // border blocks spread evenly over the launch (else all before the interior tiles)
     StepEnd end;     // first stage of a device-resident run: end.partials non-null ends the previous step
 };
 
@@ -1082,23 +1083,58 @@ template <class R> __device__ void end_previous_step(const StageArgs<R> &A, Part
     if (!verdict) sample_state(A.in, E.probes, E.nrec, E.samples, E.row);
 }
 
-template <class R, int DX, int DZ, int MECH, bool TILE = false>
+// Work of block b of a whole-grid launch: a border block index (border >= 0) or an interior block index
+// (interior >= 0).  Blocks go round-robin over the 8 XCDs, so the launch is handled in groups of 8 blocks.
+// Not spread: the border groups come first.  Spread: the gblocks / 8 border groups sit at groups floor(j T / Gb)
+// of the T groups (j < Gb), evenly between the interior groups, so the latency-bound border bodies overlap the
+// bandwidth-bound interior tiles instead of holding most of the device at the start of the launch.  Interior
+// block i keeps i % 8 == b % 8 either way, so tile_of_block places every interior tile on the same XCD.
+template <class R> __device__ inline void block_role(const StageArgs<R> &A, int b, int &border, int &interior) {
+    if (!A.spread) {
+        border = b < A.gblocks ? b : -1;
+        interior = b < A.gblocks ? -1 : b - A.gblocks;
+        return;
+    }
+    const long long g = b >> 3, Gb = A.gblocks >> 3, T = (A.gblocks + A.ninterior + 7) >> 3;
+    const int before = (int)((g * Gb + T - 1) / T), upto = (int)(((g + 1) * Gb + T - 1) / T); // border groups < g, <= g
+    if (upto > before) {
+        border = before * 8 + (b & 7), interior = -1;
+    } else {
+        border = -1, interior = (int)(g - before) * 8 + (b & 7);
+    }
+}
+
+// PART: 0 one launch for the whole grid (border blocks and interior tiles, see block_role); 1 the border blocks
+// only; 2 the interior tiles only.  Parts 1 and 2 run as two concurrent launches, each with its own register budget.
+template <class R, int DX, int DZ, int MECH, bool TILE = false, int PART = 0>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<R, MECH>))) void stage_kernel(
     StageArgs<R> A) {
     __shared__ Partial sh[THREADS];
     if (*A.status) return;
-    const int b = blockIdx.x;
-    if (A.end.partials && b == 0) { // block 0 is dispatched first: the previous step's end overlaps the launch
+    const int b = PART == 2 ? (int)blockIdx.x + A.gblocks : (int)blockIdx.x;
+    // block 0 of the launch is dispatched first: the previous step's end overlaps the rest of it
+    if (A.end.partials && (PART == 2 ? A.ngeneral == 0 && blockIdx.x == 0 : b == 0)) {
         end_previous_step(A, sh);
         __syncthreads(); // sh is used again below
     }
-    if (b < A.gblocks) {
-        if (b < A.ngeneral) general_body<R, DX, DZ, MECH>(A, b, sh);
-    } else if constexpr (TILE) {
-        __shared__ Tile<R> tile;
-        interior_body<R, DX, DZ, MECH, true>(A, tile_of_block(b - A.gblocks, A.ninterior), sh, &tile);
+    int border, interior;
+    if constexpr (PART == 0) {
+        block_role(A, b, border, interior);
     } else {
-        interior_body<R, DX, DZ, MECH>(A, tile_of_block(b - A.gblocks, A.ninterior), sh);
+        border = PART == 1 ? b : -1;
+        interior = PART == 2 ? b - A.gblocks : -1;
+    }
+    if (border >= 0) {
+        if (border < A.ngeneral) general_body<R, DX, DZ, MECH>(A, border, sh);
+    } else if constexpr (PART == 1) {
+        return;
+    } else if (interior < A.ninterior) {
+        if constexpr (TILE) {
+            __shared__ Tile<R> tile;
+            interior_body<R, DX, DZ, MECH, true>(A, tile_of_block(interior, A.ninterior), sh, &tile);
+        } else {
+            interior_body<R, DX, DZ, MECH>(A, tile_of_block(interior, A.ninterior), sh);
+        }
     }
 }
 

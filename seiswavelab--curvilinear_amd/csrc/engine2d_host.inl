@@ -79,6 +79,12 @@ template <class R> struct Engine2D {
     int nPartials = 0, nLevel2 = 0;
     static constexpr int kDirectPartials = 2048;
     int endInStage = 1; // device-resident runs end each step in the next step's first stage launch
+    // border blocks as their own launch on the side stream, beside the interior tiles: -1 automatic (FP64 grids of
+    // at least kSplitTiles interior tiles, where it was measured faster; FP32 and small grids are slower split)
+    int splitBorder = -1;
+    static constexpr int kSplitTiles = 8192;
+    bool split_border() const { return splitBorder < 0 ? !kFast<R> && nInterior >= kSplitTiles : splitBorder != 0; }
+    int spreadBorder = 0; // border blocks spread evenly over the whole-grid launch (measured: no gain)
     hipEvent_t fork[4]{}, join[4]{};
     int cur = 0, prev = 0, diag_rows = 0, registers = 0, nt = 1, metricOk = 0;
     int tileLds = 0; // elastic interior: stencil input staged through LDS tiles
@@ -365,9 +371,30 @@ template <class R> struct Engine2D {
 
     // slot = RK stage 0..3; with profiling on, one event pair brackets the four back-to-back stage launches
     // (an event between launches would itself add a dispatch gap)
+    template <int DX, int DZ, int PART> void launch_part(const StageArgs<R> &a, int blocks, hipStream_t s) {
+        switch (mech) {
+        case 0: stage_kernel<R, DX, DZ, 0, false, PART><<<blocks, THREADS, 0, s>>>(a); break;
+        case 1: stage_kernel<R, DX, DZ, 1, false, PART><<<blocks, THREADS, 0, s>>>(a); break;
+        case 2: stage_kernel<R, DX, DZ, 2, false, PART><<<blocks, THREADS, 0, s>>>(a); break;
+        default: stage_kernel<R, DX, DZ, 3, false, PART><<<blocks, THREADS, 0, s>>>(a); break;
+        }
+    }
     template <int DX, int DZ> void launch_both(const StageArgs<R> &a, int slot) {
         const int blocks = a.gblocks + nInterior;
         if (slot == 0) mark(0, true);
+        if (split_border() && !tileLds && a.ngeneral > 0 && nInterior > 0) {
+            // border blocks on the side stream beside the interior tiles; the next launch waits for both
+            check(hipEventRecord(fork[slot], stream), "2D fork");
+            check(hipStreamWaitEvent(side, fork[slot], 0), "2D fork");
+            launch_part<DX, DZ, 1>(a, a.ngeneral, side);
+            check(hipGetLastError(), "2D border launch");
+            check(hipEventRecord(join[slot], side), "2D join");
+            launch_part<DX, DZ, 2>(a, nInterior, stream);
+            check(hipGetLastError(), "2D interior launch");
+            check(hipStreamWaitEvent(stream, join[slot], 0), "2D join");
+            if (slot == 3) mark(0, false);
+            return;
+        }
         switch (mech) {
         case 0:
             if (tileLds) stage_kernel<R, DX, DZ, 0, true><<<blocks, THREADS, 0, stream>>>(a);
@@ -418,7 +445,7 @@ template <class R> struct Engine2D {
         a.status = status.as<int>();
         a.partials = partials.as<Partial>();
         a.border = borderNodes.as<int2>(), a.nborder = nBorder, a.ngeneral = nGeneral, a.gblocks = (nGeneral + 7) / 8 * 8;
-        a.ninterior = nInterior, a.rtx = rtx, a.nt = nt;
+        a.ninterior = nInterior, a.rtx = rtx, a.nt = nt, a.spread = spreadBorder;
         return a;
     }
     // Enqueue one RK4 step from buffer c; returns the buffer holding the new state.
@@ -841,6 +868,8 @@ int swl_engine2d_set_option(swl_engine2d *e, const char *name, int value) {
             else if (key == "lds_tiles") x.tileLds = value != 0;
             else if (key == "material_rows") x.m.by_row = value != 0 && x.rowsOk; // 0: read the per-node arrays
             else if (key == "end_in_stage") x.endInStage = value != 0; // 0: reduce + finalize kernels after every step
+            else if (key == "split_border") x.splitBorder = value < 0 ? -1 : value != 0; // -1: automatic
+            else if (key == "spread_border") x.spreadBorder = value != 0;
             else throw std::runtime_error("unknown engine option: " + key);
         });
     });

@@ -118,3 +118,19 @@ def test_fault_inside_pipelined_runs(prec):
     for f in range(5):
         assert np.array_equal(fa[f], fb[f]), f
     assert np.array_equal(ra["vx"], rb["vx"]) and np.array_equal(ra["vz"], rb["vz"])
+
+
+@pytest.mark.parametrize("prec", [Precision.Float64, Precision.Float32])
+@pytest.mark.parametrize("gsls", [False, True])
+def test_launch_layouts_equal(prec, gsls):
+    """The border blocks launched on the engine's second stream beside the interior tiles (split_border, automatic
+    for large FP64 grids) or spread evenly over the one launch (spread_border) change only the schedule: every
+    field, receiver sample and the energy are the same to the bit."""
+    p = grf_case(prec, gsls)
+    chunks = [40, 50]
+    print(f"[launch layouts] {prec.name} gsls={gsls}", flush=True)
+    base = run(p, {"split_border": 0, "spread_border": 0}, chunks)
+    for opts in ({"split_border": 1}, {"spread_border": 1}):
+        other = run(p, opts, chunks)
+        assert_same(other, base)
+        assert other["diag"]["energy"] == base["diag"]["energy"], opts
