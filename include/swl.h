@@ -458,9 +458,9 @@ typedef struct swl_run3d {
 } swl_run3d;
 int swl_engine3d_run(swl_engine3d *e, int first_step, int count, double dt, double frequency, double scale,
                      double source_peak, double *diag, double *samples, swl_run3d *result);
-/* HIP-event timing of the stage kernels (one event pair per step), as for 2D. out[6]: stage ms,
+/* HIP-event timing of the stage kernels (one event pair per step), as for 2D. out[7]: stage ms,
    stage launches, nodes per launch, VGPRs of the flat band's kernel, metric planes stored, material by depth
-   (0/1). */
+   (0/1), terrain band reading the z-row metrics only (0/1). */
 int swl_engine3d_profile(swl_engine3d *e, int enable);
 int swl_engine3d_kernel_timing(swl_engine3d *e, double *out);
 /* Engine tuning switches, all with identical fields (the energy sum may differ in its last bits, summation order):
@@ -470,7 +470,9 @@ int swl_engine3d_kernel_timing(swl_engine3d *e, double *out);
    "split_surface" (default 0): one launch per surface plane, each compiled for its depth.
    "lds_tiles" (default 0): the one-node-per-lane deep bands stage each field's tile and x / y halo in LDS.
    "brick_x", "brick_y", "brick_z": the deep bands' tile order within each XCD's run, bricks of x by y tiles by
-   z planes (brick_x 0 = tile columns, the default). */
+   z planes (brick_x 0 = tile columns, the default).
+   "z_row_metrics" (default 1): where the terrain band's inverse metrics 0..5 are exactly (1, 0, 0, 0, 1, 0), as the
+   fourth-order derivatives of x = i h, y = j h give them, the vector terrain band reads only metrics 6..8. */
 int swl_engine3d_set_option(swl_engine3d *e, const char *name, int value);
 
 /* ---------------- 3D session (ISimulationBackend3D, include/Seis3D/CurvilinearSolver.hpp:165-185) ---------------- */

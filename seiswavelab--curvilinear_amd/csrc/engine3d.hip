@@ -261,7 +261,11 @@ template <class R> __device__ inline R mat(const DMesh3<R> &m, const R *a, long 
 // general per-node path below), TERRAIN = deeper planes with stored metrics, FLAT = planes with identity
 // metrics and J = 1 (below the terrain transition).  The deep bands evaluate the same expressions of
 // PointKernels3D.hpp:92-112 field by field, so far fewer values are live at once.
-constexpr int SURFACE = 0, TERRAIN = 1, FLAT = 2;
+// TERRAIN_Z: the terrain band of a grid whose coordinates are x = i h, y = j h exactly and z terrain-following, so that
+// the fourth-order coordinate derivatives are (1, 0, z_xi) and (0, 1, z_eta) and six of the nine inverse metrics are
+// exactly (1, 0, 0, 0, 1, 0) (checked on the host): only the z row (metrics 6, 7, 8) is read, and each Cartesian
+// component drops its zero terms, which changes no value (at most the sign of a zero).
+constexpr int SURFACE = 0, TERRAIN = 1, FLAT = 2, TERRAIN_Z = 3;
 
 template <class R, int DX, int DY, int DZ, int BAND>
 __device__ __forceinline__ void deep_node(const Stage3<R> &A, long long q, int i, int j, int k, Partial3 &p) {
@@ -708,10 +712,10 @@ __global__ __launch_bounds__(THREADS) void band_vec_kernel(Stage3<R> A) {
 #pragma unroll
             for (int n = 0; n < NV; ++n) lambda[n] = l[n], mu[n] = u2[n], density[n] = r[n];
         }
-        R mt[BAND == TERRAIN ? 9 : 1][NV];
-        if constexpr (BAND == TERRAIN) {
+        R mt[BAND == FLAT ? 1 : 9][NV];
+        if constexpr (BAND == TERRAIN || BAND == TERRAIN_Z) {
 #pragma unroll
-            for (int a = 0; a < 9; ++a) { // read once
+            for (int a = BAND == TERRAIN_Z ? 6 : 0; a < 9; ++a) { // read once
                 const V x = vld3_nt<R, NV>(m.met + (size_t)a * m.mcount + q);
 #pragma unroll
                 for (int n = 0; n < NV; ++n) mt[a][n] = x[n];
@@ -720,6 +724,7 @@ __global__ __launch_bounds__(THREADS) void band_vec_kernel(Stage3<R> A) {
         // Cartesian component a of a derivative triple at node n: identity metrics, or deep_node's comp()
         auto cmp = [&](const R (&d)[3][NV], int a, int n) -> R {
             if constexpr (BAND == FLAT) return d[a][n];
+            else if constexpr (BAND == TERRAIN_Z) return a == 2 ? mt[8][n] * d[2][n] : d[a][n] + mt[6 + a][n] * d[2][n];
             else return mt[a][n] * d[0][n] + mt[3 + a][n] * d[1][n] + mt[6 + a][n] * d[2][n];
         };
         // DRP derivatives of field f along x, y, z at the lane's NV nodes (drp's sum order for each node)

@@ -27,6 +27,8 @@ template <class R> struct Engine3D {
     int vectorTerrain = 2, vectorFlat = 16 / (int)sizeof(R);
     int splitSurface = 0; // surface band: one launch per plane (option "split_surface"; 0: one launch for 4 planes)
     int overlapBands = 1; // surface band on the side stream beside the deep bands (option "overlap_bands")
+    int zRowOk = 0;       // the terrain band's metrics 0..5 are (1, 0, 0, 0, 1, 0) (checked at construction)
+    int zRowMetrics = 1;  // use that: read only metrics 6..8 in the terrain band (option "z_row_metrics")
     int written[2] = {0, 0}; // partial slots the last step wrote into partials[0], partials[1]
     std::vector<hipEvent_t> events;
     size_t eventsUsed = 0;
@@ -99,6 +101,18 @@ template <class R> struct Engine3D {
                     if (!ok) flat = nz, k = nz, j = ny, i = nx;
                 }
         m.flat = flat;
+        // z-row metric band: metrics 0..5 are (1, 0, 0, 0, 1, 0) exactly at every node of the planes below the surface
+        // band with stored metrics (the terrain band reads only metrics 6..8 there)
+        zRowOk = 1;
+        for (int k = 7; k < flat && k < nz - 3 && zRowOk; ++k)
+            for (int j = 3; j < ny - 3 && zRowOk; ++j)
+                for (int i = 3; i < nx - 3; ++i) {
+                    const double *mt = d.metrics + 9 * ((size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k));
+                    if (mt[0] != 1.0 || mt[1] != 0.0 || mt[2] != 0.0 || mt[3] != 0.0 || mt[4] != 1.0 || mt[5] != 0.0) {
+                        zRowOk = 0;
+                        break;
+                    }
+                }
         m.mcount = ((long long)off + (long long)ld * ny * std::max(flat, 1) + 63) / 64 * 64;
         m.met = put_layout<R>(met, d.metrics, 9, flat, (size_t)m.mcount);
         m.jac = put_layout<R>(jac, d.jacobian, 1, flat, (size_t)m.mcount);
@@ -256,6 +270,10 @@ template <class R> struct Engine3D {
     // The bands of a stage write disjoint planes and partial slots and only read the stage input, so with
     // overlapBands the surface band (few, long-latency blocks) runs on the side stream beside the deep bands;
     // the next launch on the engine stream waits for both.
+    template <int DX, int DY, int DZ, int BAND> void launch_vec(int nv, const Stage3<R> &a, hipStream_t s) {
+        if (nv == 4) band_vec_kernel<R, DX, DY, DZ, 16 / sizeof(R), BAND><<<a.nband, THREADS, 0, s>>>(a);
+        else band_vec_kernel<R, DX, DY, DZ, 2, BAND><<<a.nband, THREADS, 0, s>>>(a);
+    }
     template <int DX, int DY, int DZ> void launch(Stage3<R> a) {
         bool forked = false;
         for (int b = 0; b < 3; ++b) {
@@ -281,8 +299,8 @@ template <class R> struct Engine3D {
                     else stage3_kernel<R, DX, DY, DZ, SURFACE, false, 3><<<per, THREADS, 0, stream>>>(s);
                 }
             } else if (b == 0) stage3_kernel<R, DX, DY, DZ, SURFACE><<<a.nband, THREADS, 0, stream>>>(a);
-            else if (b == 1 && vectorTerrain == 4) band_vec_kernel<R, DX, DY, DZ, 16 / sizeof(R), TERRAIN><<<a.nband, THREADS, 0, stream>>>(a);
-            else if (b == 1 && vectorTerrain == 2) band_vec_kernel<R, DX, DY, DZ, 2, TERRAIN><<<a.nband, THREADS, 0, stream>>>(a);
+            else if (b == 1 && vectorTerrain && zRowOk && zRowMetrics) launch_vec<DX, DY, DZ, TERRAIN_Z>(vectorTerrain, a, stream);
+            else if (b == 1 && vectorTerrain) launch_vec<DX, DY, DZ, TERRAIN>(vectorTerrain, a, stream);
             else if (b == 1 && a.lds) stage3_kernel<R, DX, DY, DZ, TERRAIN, true><<<a.nband, THREADS, 0, stream>>>(a);
             else if (b == 1) stage3_kernel<R, DX, DY, DZ, TERRAIN><<<a.nband, THREADS, 0, stream>>>(a);
             else if (vectorFlat == 4) band_vec_kernel<R, DX, DY, DZ, 16 / sizeof(R), FLAT><<<a.nband, THREADS, 0, stream>>>(a);
@@ -566,6 +584,7 @@ int swl_engine3d_set_option(swl_engine3d *e, const char *name, int value) {
             else if (key == "brick_z") x.brick[2] = std::max(1, value);
             else if (key == "split_surface") x.splitSurface = value != 0;
             else if (key == "overlap_bands") x.overlapBands = value != 0;
+            else if (key == "z_row_metrics") x.zRowMetrics = value != 0;
             else if (key == "vector_flat" || key == "vector_terrain") {
                 using T = std::remove_reference_t<decltype(x)>;
                 swl::require(value == 0 || value == 1 || value == 2 || (value == 4 && T::real_bytes == 4),
@@ -582,6 +601,7 @@ int swl_engine3d_kernel_timing(swl_engine3d *e, double *out) {
             out[0] = x.stageMs, out[1] = (double)x.stageLaunches;
             out[2] = (double)(x.nx - 6) * (double)(x.ny - 6) * (double)(x.nz - 6);
             out[3] = x.flatRegisters(), out[4] = x.m.flat, out[5] = x.m.by_depth;
+            out[6] = x.zRowOk && x.zRowMetrics && x.vectorTerrain;
         });
     });
 }

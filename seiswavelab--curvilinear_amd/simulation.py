@@ -567,10 +567,10 @@ class Simulation3D:
 
     def kernel_timing(self) -> dict:
         self._need()
-        o = np.zeros(6)
+        o = np.zeros(7)
         native.call("swl_sim3d_kernel_timing", self._h, _dp(o))
         return {"stage_ms": o[0], "stage_launches": int(o[1]), "nodes_per_launch": int(o[2]), "vgprs": int(o[3]),
-                "metric_planes": int(o[4]), "material_by_depth": bool(o[5])}
+                "metric_planes": int(o[4]), "material_by_depth": bool(o[5]), "z_row_metrics": bool(o[6])}
 
 
 class AsyncSimulation3D:

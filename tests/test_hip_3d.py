@@ -218,17 +218,23 @@ def _odd_case(name):
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("name", ["flat_datum_layers", "analytic_sponge_force"])
 def test_vector_bands_equal_node_lanes_3d(name, precision):
-    """The deep bands with 2 or 4 x-adjacent nodes per lane (options "vector_terrain", "vector_flat") compute every
-    node with the one-node-per-lane expressions: fields and traces bit-identical in every combination; the energy
+    """The deep bands with 2 or 4 x-adjacent nodes per lane (options "vector_terrain", "vector_flat"), with the terrain
+    band reading all nine metrics or only their z row ("z_row_metrics", where the other six are exactly 1, 0, 0, 0,
+    1, 0), compute every node with the one-node-per-lane expressions: fields and traces bit-identical in every
+    combination; the energy
     (a sum over tiles of another shape) to 1e-12.  analytic_sponge_force has a terrain band (planes 7..14) and a
     flat band."""
     p = _odd_case(name)
     runs = []
-    widths = ((0, 0), (2, 2), (2, 0), (0, 2)) if precision == 0 else ((0, 0), (2, 4), (4, 2), (4, 0), (0, 4))
-    for terrain, flat in widths:
+    # (terrain, flat, z_row_metrics): z_row_metrics reads only the z row of the terrain band's metrics
+    widths = (((0, 0, 1), (2, 2, 1), (2, 0, 1), (0, 2, 1), (2, 2, 0)) if precision == 0 else
+              ((0, 0, 1), (2, 4, 1), (4, 2, 1), (4, 0, 1), (0, 4, 1), (2, 4, 0), (4, 2, 0)))
+    for terrain, flat, zrow in widths:
         s = _gpu(p, precision)
         s.set_option("vector_terrain", terrain)
         s.set_option("vector_flat", flat)
+        s.set_option("z_row_metrics", zrow)
+        assert s.kernel_timing()["z_row_metrics"] == bool(terrain and zrow), (terrain, zrow)
         t = s.kernel_timing()
         assert max(7, t["metric_planes"]) <= p.nz + 3 - 4, "the case has no flat band of 4 planes"
         ok, err = s.advance(p.maxSteps)
