@@ -472,7 +472,8 @@ int swl_engine3d_kernel_timing(swl_engine3d *e, double *out);
    "brick_x", "brick_y", "brick_z": the deep bands' tile order within each XCD's run, bricks of x by y tiles by
    z planes (brick_x 0 = tile columns, the default).
    "z_row_metrics" (default 1): where the terrain band's inverse metrics 0..5 are exactly (1, 0, 0, 0, 1, 0), as the
-   fourth-order derivatives of x = i h, y = j h give them, the vector terrain band reads only metrics 6..8. */
+   fourth-order derivatives of x = i h, y = j h give them, the vector terrain band reads only metrics 6..8.
+   "z_row_surface" (default 1): there, the surface band takes metrics 0..5 as the constants they are. */
 int swl_engine3d_set_option(swl_engine3d *e, const char *name, int value);
 
 /* ---------------- 3D session (ISimulationBackend3D, include/Seis3D/CurvilinearSolver.hpp:165-185) ---------------- */

@@ -175,29 +175,35 @@ template <class R, int DIR> __device__ inline R low2(const R *__restrict__ v, lo
 template <class R> __device__ inline R metric(const DMesh3<R> &m, long long q, int k, int a) {
     return k < m.flat ? m.met[(size_t)a * m.mcount + q] : (a % 4 == 0 ? (R)1 : (R)0);
 }
+// ZR: metrics 0..5 are known to be (1, 0, 0, 0, 1, 0) at every stored plane (TERRAIN_Z below): constants instead of
+// loads, the same values in the same operations
+template <class R, bool ZR> __device__ inline R metric_z(const DMesh3<R> &m, long long q, int k, int a) {
+    if (ZR && a < 6) return a % 4 == 0 ? (R)1 : (R)0;
+    return metric(m, q, k, a);
+}
 template <class R> __device__ inline R jacobian(const DMesh3<R> &m, long long q, int k) {
     return k < m.flat ? m.jac[q] : (R)1;
 }
 
 // traction flux J*(grad axis).sigma (PointKernels3D.hpp:67-72)
-template <class R>
+template <class R, bool ZR = false>
 __device__ inline R tflux(const DMesh3<R> &m, const R *__restrict__ w, size_t count, long long q, int k, int axis, int comp) {
     const int f0 = comp == 0 ? 3 : comp == 1 ? 6 : 7, f1 = comp == 0 ? 6 : comp == 1 ? 4 : 8, f2 = comp == 0 ? 7 : comp == 1 ? 8 : 5;
     R v = 0;
-    v += metric(m, q, k, 3 * axis + 0) * w[(size_t)f0 * count + q];
-    v += metric(m, q, k, 3 * axis + 1) * w[(size_t)f1 * count + q];
-    v += metric(m, q, k, 3 * axis + 2) * w[(size_t)f2 * count + q];
+    v += metric_z<R, ZR>(m, q, k, 3 * axis + 0) * w[(size_t)f0 * count + q];
+    v += metric_z<R, ZR>(m, q, k, 3 * axis + 1) * w[(size_t)f1 * count + q];
+    v += metric_z<R, ZR>(m, q, k, 3 * axis + 2) * w[(size_t)f2 * count + q];
     return jacobian(m, q, k) * v;
 }
 // surfaceMomentum (PointKernels3D.hpp:73-91)
-template <class R, int DX, int DY, int DZ>
+template <class R, int DX, int DY, int DZ, bool ZR = false>
 __device__ R smomentum(const DMesh3<R> &m, const R *__restrict__ w, size_t count, long long q, int k, int comp, int depth,
                        R rho) {
     R sum = 0;
 #pragma unroll
-    for (int l = 0; l < 5; ++l) sum += cf<R, DX>(l) * tflux(m, w, count, q + of<DX>(l), k, 0, comp);
+    for (int l = 0; l < 5; ++l) sum += cf<R, DX>(l) * tflux<R, ZR>(m, w, count, q + of<DX>(l), k, 0, comp);
 #pragma unroll
-    for (int l = 0; l < 5; ++l) sum += cf<R, DY>(l) * tflux(m, w, count, q + of<DY>(l) * (long long)m.ld, k, 1, comp);
+    for (int l = 0; l < 5; ++l) sum += cf<R, DY>(l) * tflux<R, ZR>(m, w, count, q + of<DY>(l) * (long long)m.ld, k, 1, comp);
 #pragma unroll
     for (int l = 0; l < 5; ++l) {
         int o = of<DZ>(l);
@@ -205,7 +211,7 @@ __device__ R smomentum(const DMesh3<R> &m, const R *__restrict__ w, size_t count
         const int z = depth + o;
         if (z == 0) continue;
         if (z < 0) o = -z - depth, sign = -1;
-        sum += sign * cf<R, DZ>(l) * tflux(m, w, count, q + o * m.plane, k + o, 2, comp);
+        sum += sign * cf<R, DZ>(l) * tflux<R, ZR>(m, w, count, q + o * m.plane, k + o, 2, comp);
     }
     return sum / (m.h * jacobian(m, q, k) * rho);
 }
@@ -516,7 +522,7 @@ __device__ __forceinline__ void deep_node_lds(const Stage3<R> &A, long long q0, 
     }
 }
 
-template <class R, int DX, int DY, int DZ, int BAND, bool LDS = false, int SD = -1>
+template <class R, int DX, int DY, int DZ, int BAND, bool LDS = false, int SD = -1, bool ZR = false>
 __global__ __launch_bounds__(THREADS) void stage3_kernel(Stage3<R> A) {
     if (*A.status) return;
     const DMesh3<R> &m = A.m;
@@ -569,7 +575,7 @@ __global__ __launch_bounds__(THREADS) void stage3_kernel(Stage3<R> A) {
         R grad[9][3];
         R mt[9];
 #pragma unroll
-        for (int a = 0; a < 9; ++a) mt[a] = metric(m, q, k, a);
+        for (int a = 0; a < 9; ++a) mt[a] = metric_z<R, ZR>(m, q, k, a);
 #pragma unroll
         for (int f = 0; f < 9; ++f) {
             const R *v = w + (size_t)f * cnt;
@@ -588,7 +594,7 @@ __global__ __launch_bounds__(THREADS) void stage3_kernel(Stage3<R> A) {
         kv[1] = (grad[6][0] + grad[4][1] + grad[8][2]) / density;
         kv[2] = (grad[7][0] + grad[8][1] + grad[5][2]) / density;
         if (depth <= (DZ ? 3 : 1))
-            for (int a = 0; a < 3; ++a) kv[a] = smomentum<R, DX, DY, DZ>(m, w, cnt, q, k, a, depth, density);
+            for (int a = 0; a < 3; ++a) kv[a] = smomentum<R, DX, DY, DZ, ZR>(m, w, cnt, q, k, a, depth, density);
         if (m.si1 >= 0 && i >= m.si0 && i <= m.si1 && j >= m.sj0 && j <= m.sj1 && k >= m.sk0 && k <= m.sk1) {
             for (int a = 0; a < 3; ++a)
                 kv[a] += A.forcing * (m.grad[0] ? m.grad[a][q] : m.src[q] * m.dir[a]) * (m.rho_global / density);

@@ -29,6 +29,7 @@ template <class R> struct Engine3D {
     int overlapBands = 1; // surface band on the side stream beside the deep bands (option "overlap_bands")
     int zRowOk = 0;       // the terrain band's metrics 0..5 are (1, 0, 0, 0, 1, 0) (checked at construction)
     int zRowMetrics = 1;  // use that: read only metrics 6..8 in the terrain band (option "z_row_metrics")
+    int zRowSurface = 1;  // and take metrics 0..5 as constants in the surface band (option "z_row_surface")
     int written[2] = {0, 0}; // partial slots the last step wrote into partials[0], partials[1]
     std::vector<hipEvent_t> events;
     size_t eventsUsed = 0;
@@ -101,10 +102,10 @@ template <class R> struct Engine3D {
                     if (!ok) flat = nz, k = nz, j = ny, i = nx;
                 }
         m.flat = flat;
-        // z-row metric band: metrics 0..5 are (1, 0, 0, 0, 1, 0) exactly at every node of the planes below the surface
-        // band with stored metrics (the terrain band reads only metrics 6..8 there)
+        // z-row metric band: metrics 0..5 are (1, 0, 0, 0, 1, 0) exactly at every node of every plane with stored
+        // metrics (the terrain band then reads only metrics 6..8, the surface band takes 0..5 as constants)
         zRowOk = 1;
-        for (int k = 7; k < flat && k < nz - 3 && zRowOk; ++k)
+        for (int k = 3; k < flat && k < nz - 3 && zRowOk; ++k)
             for (int j = 3; j < ny - 3 && zRowOk; ++j)
                 for (int i = 3; i < nx - 3; ++i) {
                     const double *mt = d.metrics + 9 * ((size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k));
@@ -298,6 +299,8 @@ template <class R> struct Engine3D {
                     else if (depth == 2) stage3_kernel<R, DX, DY, DZ, SURFACE, false, 2><<<per, THREADS, 0, stream>>>(s);
                     else stage3_kernel<R, DX, DY, DZ, SURFACE, false, 3><<<per, THREADS, 0, stream>>>(s);
                 }
+            } else if (b == 0 && zRowOk && zRowSurface) {
+                stage3_kernel<R, DX, DY, DZ, SURFACE, false, -1, true><<<a.nband, THREADS, 0, stream>>>(a);
             } else if (b == 0) stage3_kernel<R, DX, DY, DZ, SURFACE><<<a.nband, THREADS, 0, stream>>>(a);
             else if (b == 1 && vectorTerrain && zRowOk && zRowMetrics) launch_vec<DX, DY, DZ, TERRAIN_Z>(vectorTerrain, a, stream);
             else if (b == 1 && vectorTerrain) launch_vec<DX, DY, DZ, TERRAIN>(vectorTerrain, a, stream);
@@ -585,6 +588,7 @@ int swl_engine3d_set_option(swl_engine3d *e, const char *name, int value) {
             else if (key == "split_surface") x.splitSurface = value != 0;
             else if (key == "overlap_bands") x.overlapBands = value != 0;
             else if (key == "z_row_metrics") x.zRowMetrics = value != 0;
+            else if (key == "z_row_surface") x.zRowSurface = value != 0;
             else if (key == "vector_flat" || key == "vector_terrain") {
                 using T = std::remove_reference_t<decltype(x)>;
                 swl::require(value == 0 || value == 1 || value == 2 || (value == 4 && T::real_bytes == 4),

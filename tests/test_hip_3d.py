@@ -234,6 +234,7 @@ def test_vector_bands_equal_node_lanes_3d(name, precision):
         s.set_option("vector_terrain", terrain)
         s.set_option("vector_flat", flat)
         s.set_option("z_row_metrics", zrow)
+        s.set_option("z_row_surface", zrow)
         assert s.kernel_timing()["z_row_metrics"] == bool(terrain and zrow), (terrain, zrow)
         t = s.kernel_timing()
         assert max(7, t["metric_planes"]) <= p.nz + 3 - 4, "the case has no flat band of 4 planes"
