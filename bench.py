@@ -227,7 +227,7 @@ ALGO_BYTES_3D = {"f32": 784.0, "f64": 1568.0}  # per node per RK4 step, 3D nine-
 def measure_3d(pkg, steps, warmup, precision):
     """400x288x200 DEM-terrain 3D elastic shot; FP64 (precision 0) or FP32 (1)."""
     from seiswavelab_curvilinear_amd import workloads
-    p, seed = workloads.dem_terrain_400x288x200(steps=max(1500, warmup + steps))
+    p, seed = workloads.dem_terrain_400x288x200(steps=max(1500, warmup + steps + max(4, min(steps, 20))))
     sim = pkg.Simulation3D(pkg.BackendKind.Hip)
     t = time.perf_counter()
     ok, err = sim.initialize(p, 0, precision)
@@ -236,10 +236,13 @@ def measure_3d(pkg, steps, warmup, precision):
     label = "FP64" if precision == 0 else "FP32"
     log(f"[bench] 3D {label}: initialized in {time.perf_counter() - t:.1f} s, dt={sim.dt():.6g} s")
     advance_exactly(sim, warmup, f"3D {label}")
-    sim.profile(True)
+    sim.profile(False)
     t0 = time.perf_counter()
     steps = advance_exactly(sim, steps, f"3D {label}")
     wall = time.perf_counter() - t0
+    # per-launch timing with HIP events over steps run after the timed ones (as for 2D)
+    sim.profile(True)
+    advance_exactly(sim, max(4, min(steps, 20)), f"3D {label}")
     timing = sim.kernel_timing()
     d = sim.diagnostics()
     sim.close()
