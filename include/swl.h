@@ -208,15 +208,24 @@ int swl_engine2d_run_collect(swl_engine2d *e, int slot, double *diag, double *sa
    back-to-back stage launches of a step; the finalisation is timed on every 32nd step.
    enable resets the counters. */
 int swl_engine2d_profile(swl_engine2d *e, int enable);
-/* out[10]: stage-kernel ms, stage launches, finalisation ms, finalisation launches, interior tiles,
-   stage-kernel VGPRs, computational nodes per stage launch, border tiles, on-the-fly metrics (0/1),
-   per-row material tables (1: lambda, mu, rho, dP, dS depend on the row only; 0: per-node arrays). */
+/* out[12]: stage-kernel ms, stage launches, finalisation ms, finalisation launches, interior tiles,
+   stage-kernel VGPRs, computational nodes per stage launch, border tiles (general blocks), on-the-fly metrics
+   (0/1), per-row material tables (1: lambda, mu, rho, dP, dS depend on the row only; 0: per-node arrays),
+   PML vector lanes (two x-adjacent nodes each), PML blocks. */
 int swl_engine2d_kernel_timing(swl_engine2d *e, double *out);
 /* Engine tuning switches: "nontemporal_stores" (default 1) for the interior outputs;
    "stored_metrics" (default 0) reads a, b, J from memory instead of recomputing them;
    "lds_tiles" (default 0) stages the elastic interior's stencil input through LDS;
    "material_rows" (default 1 where the materials depend on the row only) reads the per-row tables,
-   0 reads the per-node arrays. */
+   0 reads the per-node arrays;
+   "pml_vector" (-1 automatic: grids of at least 1024 interior tiles; 0 off; 1 on): FP32 elastic CFS-PML models
+   update the absorbing strips below the surface band (depth >= 4, no sponge) as lanes of two x-adjacent nodes
+   instead of one node per lane (same equations, rounded differently: FP32 tolerance, not bit-identical).
+   "end_in_stage" (default 1): step j is ended (reduction, fault rule, diagnostics and receiver rows) by block 0
+   of step j + 1's first stage launch; 0 launches a reduction and a finalisation kernel after every step.
+   "split_border" (-1 automatic: large FP64 grids; 0 off; 1 on): the border blocks run as their own launch on the
+   engine's second stream beside the interior tiles.  "spread_border" (default 0): the border blocks are spread
+   evenly over the one launch instead of dispatched first.  These three change the schedule only. */
 int swl_engine2d_set_option(swl_engine2d *e, const char *name, int value);
 /* Measurement utility (no reference counterpart): `repeats` plain streaming copies of `bytes`
    (read `bytes`, write `bytes`) on `device`; ms_per_copy = mean HIP-event time per copy.  Used to

@@ -39,7 +39,7 @@ template <class R> struct BatchTyped {
 
     void build_groups(const std::vector<Engine2D<R> *> &engines) {
         std::vector<uint64_t> ids;
-        for (auto *e : engines) ids.push_back(e->uid);
+        for (auto *e : engines) ids.push_back(e->uid), ids.push_back((uint64_t)e->nGeneral << 32 | (uint32_t)e->nPmlBlocks);
         if (ids == key) return;
         key = ids;
         groups.clear();
@@ -53,8 +53,9 @@ template <class R> struct BatchTyped {
             // interior tiles in the XCD-band order of the single-shot kernel
             for (int local = 0; local < (int)g.members.size(); ++local) {
                 const Engine2D<R> &e = *engines[g.members[local]];
-                const int gblocks = (e.nGeneral + 7) / 8 * 8;
+                const int gblocks = (e.nGeneral + 7) / 8 * 8, pblocks = (e.nPmlBlocks + 7) / 8 * 8;
                 for (int b = 0; b < gblocks; ++b) g.work.push_back(int2{local, b < e.nGeneral ? b : -1});
+                for (int b = 0; b < pblocks; ++b) g.work.push_back(int2{local, b < e.nPmlBlocks ? (b | BATCH_PML) : -1});
                 const int n = e.nInterior, padded = (n + 7) / 8 * 8;
                 for (int b = 0; b < padded; ++b) {
                     // tile_of_block(b, n): position b (XCD b % 8) takes tile x*q + min(x, r) + j, a bijection of [0, n)

@@ -112,6 +112,9 @@ template <class R> struct StageArgs {
     const int2 *border;      // (i, k) of every node outside the interior rectangle, row by row, 256 per block
     int nborder;             // border nodes
     int ngeneral, gblocks, ninterior, rtx;
+    const int2 *pml;  // FP32 elastic: (i0, k) of every CFS-PML vector lane below the surface band, 256 per block
+    int npml, pml0, npmlb; // lanes, first border-class block index and number of their blocks
+    int ipart0;       // partial slot of interior tile 0 (after the general and PML blocks' slots)
     int nt;          // interior: nontemporal stores
     int spread;      // border blocks spread evenly over the launch (else all before the interior tiles)
     StepEnd end;     // first stage of a device-resident run: end.partials non-null ends the previous step
@@ -1033,7 +1036,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
     }
     if (A.kind == 2) {
         block_reduce(p, sh);
-        if (threadIdx.x == 0) A.partials[A.ngeneral + tl] = sh[0];
+        if (threadIdx.x == 0) A.partials[A.ipart0 + tl] = sh[0];
     }
 }
 
@@ -1083,6 +1086,183 @@ template <class R> __device__ void end_previous_step(const StageArgs<R> &A, Part
     if (!verdict) sample_state(A.in, E.probes, E.nrec, E.samples, E.row);
 }
 
+// FP32 elastic CFS-PML strips below the surface band (depth >= 4: no closure, traction image or reduced stencil;
+// no sponge, checked on the host).  Each lane owns NV x-adjacent nodes as in the interior (vector
+// stencil loads, metrics recomputed from the 1D profiles), and each node's four ADE memories per axis, contiguous in
+// the compact slot map (PointKernels2D.hpp:50-56), are read and written as one vector.  The arithmetic is
+// general_body's for these nodes (PointKernels2D.hpp:121-172 with the stretched derivatives of :57-64) in FP32 with
+// fused multiply-adds; the wavefields use the interior's accumulator form, the memories general_body's.
+constexpr int PML_NV = 2; // nodes per PML lane (8-byte vectors: half the interior's live stencil values)
+template <class R, int DX, int DZ> __device__ __forceinline__ void pml_body(const StageArgs<R> &A, int li, Partial *sh) {
+    static_assert(kFast<R>, "FP32 only");
+    constexpr int NV = PML_NV, HV = (3 + NV - 1) / NV;
+    typedef R V __attribute__((ext_vector_type(NV)));
+    using V4 = typename Lane<R>::V; // one node's four ADE memories
+    const auto vl = [](const R *p) { return *reinterpret_cast<const V *>(p); };
+    const DMesh<R> &m = A.m;
+    const int n = li * THREADS + (int)threadIdx.x;
+    Partial p{0, 0, 0, 0, 0};
+    if (n < A.npml) {
+        const int2 ik = A.pml[n];
+        const int i0 = ik.x, k = ik.y, valid = min(NV, m.nx - 3 - i0);
+        const ptrdiff_t q = at(m, i0, k), s = m.ld;
+        const size_t cnt = A.count;
+        const R h = m.h;
+        R xi[5][NV], eta[5][NV];
+        V centre[5];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+            const R *fp = A.in + (size_t)f * cnt;
+            R row[(2 * HV + 1) * NV];
+#pragma unroll
+            for (int v = 0; v <= 2 * HV; ++v) {
+                const V x = vl(fp + q + (v - HV) * NV);
+                if (v == HV) centre[f] = x;
+#pragma unroll
+                for (int j = 0; j < NV; ++j) row[v * NV + j] = x[j];
+            }
+#pragma unroll
+            for (int j = 0; j < NV; ++j) {
+                R r = 0;
+#pragma unroll
+                for (int l = 0; l < 5; ++l) r = mad(coef<R, DX>(l), row[HV * NV + j + offs<DX>(l)], r);
+                xi[f][j] = over_h(r, h, m.hinv);
+            }
+            V col[5];
+#pragma unroll
+            for (int l = 0; l < 5; ++l) col[l] = offs<DZ>(l) == 0 ? centre[f] : vl(fp + q + offs<DZ>(l) * s);
+#pragma unroll
+            for (int j = 0; j < NV; ++j) {
+                R r = 0;
+#pragma unroll
+                for (int l = 0; l < 5; ++l) r = mad(coef<R, DZ>(l), col[l][j], r);
+                eta[f][j] = over_h(r, h, m.hinv);
+            }
+        }
+        R a[NV], b[NV], jac[NV];
+        if (m.metric) {
+            const double fk = m.fade[k], fdk = m.fadeder[k];
+#pragma unroll
+            for (int j = 0; j < NV; ++j) {
+                const double zx = m.tslope[i0 + j] * fk, jj = 1 + m.tz_[i0 + j] * fdk, inv = 1 / jj;
+                a[j] = (R)(-zx * inv), b[j] = (R)inv, jac[j] = (R)jj;
+            }
+        } else {
+            const V va = vl(m.a + q), vb = vl(m.b + q), vj = vl(m.jac + q);
+#pragma unroll
+            for (int j = 0; j < NV; ++j) a[j] = va[j], b[j] = vb[j], jac[j] = vj[j];
+        }
+        const V vrho = m.by_row ? V(m.rho_k[k]) : vl(m.rho + q), vlam = m.by_row ? V(m.lam_k[k]) : vl(m.lam + q),
+                vmu = m.by_row ? V(m.mu_k[k]) : vl(m.mu + q);
+        const R *fz = m.czf + 3 * (k - 3);
+        const bool src = m.si1 >= 0 && k >= m.sk0 && k <= m.sk1 && i0 + NV - 1 >= m.si0 && i0 <= m.si1;
+        R kv[5][NV];
+        bool bad = false;
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            const int i = i0 + j;
+            const long long sx = pml_slot(m.nx, m.nz, m.wx, m.wz, m.cpml, i, k, 0),
+                            sz = pml_slot(m.nx, m.nz, m.wx, m.wz, m.cpml, i, k, 1);
+            const bool ox = j < valid && sx >= 0, oz = j < valid && sz >= 0;
+            const V4 px = ox ? vload(A.pin + sx) : V4{}, pz = oz ? vload(A.pin + sz) : V4{};
+            const R *fx = m.cxf + 3 * (i - 3);
+            R rx[4], rz[4]; // memory rates: fields 0, 1 (velocity derivatives) and 2, 3 (momentum parts)
+            // stretched derivative (PointKernels2D.hpp:57-64) with the profile's quotients per position
+            const auto stretch = [](bool on, const R *f, R psi, R d, R &rate) -> R {
+                if (!on) return rate = 0, d;
+                rate = -mad(f[1], psi, f[2] * d);
+                return mad(d, f[0], psi);
+            };
+            const R vx0 = stretch(ox, fx, px[0], xi[0][j], rx[0]), vx1 = stretch(ox, fx, px[1], xi[1][j], rx[1]);
+            const R vz0 = stretch(oz, fz, pz[0], eta[0][j], rz[0]), vz1 = stretch(oz, fz, pz[1], eta[1][j], rz[1]);
+            const R dx0 = mad(a[j], vz0, vx0), dz0 = b[j] * vz0, dx1 = mad(a[j], vz1, vx1), dz1 = b[j] * vz1;
+            const R rinv = (R)1 / vrho[j];
+            const R x2 = xi[2][j] * rinv, z2 = mad(a[j], eta[2][j], b[j] * eta[4][j]) * rinv;
+            const R x3 = xi[4][j] * rinv, z3 = mad(a[j], eta[4][j], b[j] * eta[3][j]) * rinv;
+            kv[0][j] = stretch(ox, fx, px[2], x2, rx[2]) + stretch(oz, fz, pz[2], z2, rz[2]);
+            kv[1][j] = stretch(ox, fx, px[3], x3, rx[3]) + stretch(oz, fz, pz[3], z3, rz[3]);
+            if (src) { // the body force where the lane meets the source's support (as in the interior)
+                const ptrdiff_t qq = q + j;
+                const R g0 = m.has_grad ? m.grad0[qq] : m.src[qq] * m.dir0;
+                const R g1 = m.has_grad ? m.grad1[qq] : m.src[qq] * m.dir2;
+                kv[0][j] = mad(A.forcing * g0, rinv, kv[0][j]);
+                kv[1][j] = mad(A.forcing * g1, rinv, kv[1][j]);
+            }
+            const R lambda = vlam[j], mu = vmu[j], div = dx0 + dz1;
+            kv[2][j] = mad(lambda, div, 2 * mu * dx0);
+            kv[3][j] = mad(lambda, div, 2 * mu * dz1);
+            kv[4][j] = mu * (dz0 + dx1);
+            // the node's memories: combine (CurvilinearSolver2D.cpp:80-97) with general_body's accumulator form
+#pragma unroll
+            for (int axis = 0; axis < 2; ++axis) {
+                if (!(axis ? oz : ox)) continue;
+                const long long sl = axis ? sz : sx;
+                const R *rate = axis ? rz : rx;
+                const V4 pb = A.kind == 0 ? (axis ? pz : px) : vload(A.pbase + sl);
+                const V4 pa = A.kind == 0 ? V4{} : vload(A.pacc + sl);
+                V4 pn, pacc;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    R acc = pa[c];
+                    pn[c] = combine(A.kind, pb[c], rate[c], A.w, acc);
+                    pacc[c] = acc;
+                    if (A.kind == 2 && !(pn[c] <= (R)3.4028234663852886e38 && pn[c] >= (R)-3.4028234663852886e38)) bad = true;
+                }
+                vstore(A.pout + sl, pn, 0);
+                if (A.kind != 2) vstore(A.pacc + sl, pacc, 0);
+            }
+        }
+        // the wavefields (interior_body's FP32 accumulator form); a lane past the grid's last column stores its nodes
+        // one by one
+        R nxt[5][NV];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+            const size_t o = (size_t)f * cnt + q;
+            const V vin = centre[f], vb = A.kind == 0 ? vin : A.kind == 2 ? V{} : vl(A.base + o);
+            const V vy = A.kind == 0 ? vb : A.ystage ? V{} : vl(A.acc + o);
+            V vo, va;
+#pragma unroll
+            for (int j = 0; j < NV; ++j) {
+                const R y = A.ystage ? mad(vin[j] - vb[j], (R)(1.0 / 3.0), vb[j]) : vy[j];
+                if (A.kind == 2) {
+                    nxt[f][j] = mad(A.wy, kv[f][j], y);
+                } else {
+                    nxt[f][j] = mad(A.w, kv[f][j], vb[j]);
+                    va[j] = mad(A.wy, kv[f][j], y);
+                }
+                vo[j] = nxt[f][j];
+            }
+            if (valid == NV) {
+                *reinterpret_cast<V *>(A.out + o) = vo;
+                if (A.kind == 1) *reinterpret_cast<V *>(A.acc + o) = va;
+            } else {
+                for (int j = 0; j < valid; ++j) {
+                    A.out[o + j] = vo[j];
+                    if (A.kind == 1) A.acc[o + j] = va[j];
+                }
+            }
+        }
+        if (A.kind == 2) {
+#pragma unroll
+            for (int j = 0; j < NV; ++j) {
+                if (j >= valid) continue;
+                R u[5];
+#pragma unroll
+                for (int f = 0; f < 5; ++f) {
+                    u[f] = nxt[f][j];
+                    if (!isfinite(u[f])) bad = true;
+                }
+                node_stats<R, true>(vrho[j], vlam[j], vmu[j], jac[j], h, u, false, nullptr, p);
+            }
+            if (bad) p.bad = 1;
+        }
+    }
+    if (A.kind == 2) {
+        block_reduce(p, sh);
+        if (threadIdx.x == 0) A.partials[A.ngeneral + li] = sh[0];
+    }
+}
+
 // Work of block b of a whole-grid launch: a border block index (border >= 0) or an interior block index
 // (interior >= 0).  Blocks go round-robin over the 8 XCDs, so the launch is handled in groups of 8 blocks.
 // Not spread: the border groups come first.  Spread: the gblocks / 8 border groups sit at groups floor(j T / Gb)
@@ -1113,7 +1293,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(kWavesP
     if (*A.status) return;
     const int b = PART == 2 ? (int)blockIdx.x + A.gblocks : (int)blockIdx.x;
     // block 0 of the launch is dispatched first: the previous step's end overlaps the rest of it
-    if (A.end.partials && (PART == 2 ? A.ngeneral == 0 && blockIdx.x == 0 : b == 0)) {
+    if (A.end.partials && (PART == 2 ? A.gblocks == 0 && blockIdx.x == 0 : b == 0)) {
         end_previous_step(A, sh);
         __syncthreads(); // sh is used again below
     }
@@ -1125,7 +1305,11 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(kWavesP
         interior = PART == 2 ? b - A.gblocks : -1;
     }
     if (border >= 0) {
-        if (border < A.ngeneral) general_body<R, DX, DZ, MECH>(A, border, sh);
+        if (border < A.ngeneral) {
+            general_body<R, DX, DZ, MECH>(A, border, sh);
+        } else if constexpr (kFast<R> && MECH == 0) {
+            if (border >= A.pml0 && border < A.pml0 + A.npmlb) pml_body<R, DX, DZ>(A, border - A.pml0, sh);
+        }
     } else if constexpr (PART == 1) {
         return;
     } else if (interior < A.ninterior) {
@@ -1174,14 +1358,19 @@ template <class R> __global__ __launch_bounds__(THREADS) void finalize_kernel(Fi
 // (already XCD-band mapped), -1 padding.  args[shot] are the shot's stage arguments for this launch;
 // kind < 0 marks a shot that has no work in it (finished, or another direction pair).  Grid-stride
 // loop, so the grid may be smaller than the work list (persistent blocks).
-constexpr int BATCH_INTERIOR = 1 << 30;
+constexpr int BATCH_INTERIOR = 1 << 30, BATCH_PML = 1 << 29;
 template <class R, int DX, int DZ, int MECH>
 __device__ __forceinline__ void batch_item(const StageArgs<R> *__restrict__ args, const int2 w, Partial *sh) {
     if (w.y < 0) return;
     const StageArgs<R> &A = args[w.x];
     if (A.kind < 0 || *A.status) return;
-    if (w.y & BATCH_INTERIOR) interior_body<R, DX, DZ, MECH>(A, w.y & ~BATCH_INTERIOR, sh);
-    else general_body<R, DX, DZ, MECH>(A, w.y, sh);
+    if (w.y & BATCH_INTERIOR) {
+        interior_body<R, DX, DZ, MECH>(A, w.y & ~BATCH_INTERIOR, sh);
+    } else if (w.y & BATCH_PML) {
+        if constexpr (kFast<R> && MECH == 0) pml_body<R, DX, DZ>(A, w.y & ~BATCH_PML, sh);
+    } else {
+        general_body<R, DX, DZ, MECH>(A, w.y, sh);
+    }
 }
 // PERSIST = 0: one block per work item (grid = nwork); 1: grid-stride loop over the items
 template <class R, int DX, int DZ, int MECH, int PERSIST>

@@ -72,9 +72,17 @@ template <class R> struct Engine2D {
     // the next step's first stage writes neither c + 2 nor c, so that stage can end the step before it (StepEnd)
     static constexpr int NB = 4;
     DeviceBuffer state[NB], accbuf, pstate[NB], paccbuf, partials, level2, probes, status, accepted, peak, diag, samples,
-        gather, display, borderNodes;
+        gather, display, borderNodes, pmlLanes;
     int ntx = 0, ntiles = 0;      // 64x4 tiles over the computational grid
-    int nBorder = 0, nGeneral = 0; // nodes outside the interior rectangle, blocks of 256 of them
+    int nBorder = 0, nGeneral = 0; // nodes outside the interior rectangle (and the PML lanes), blocks of 256 of them
+    int nPml = 0, nPmlBlocks = 0;  // FP32 elastic: CFS-PML vector lanes below the surface band, blocks of 256 of them
+    int gBlocks = 0;               // border-class blocks of a stage launch (general, then PML), a multiple of 8
+    // PML vector lanes (option "pml_vector"): -1 automatic, 0 off, 1 on; the lists are rebuilt when the choice changes.
+    // Automatic: grids of at least kPmlTiles interior tiles, where they measured 2-5 % faster per step (fewer,
+    // longer border threads lengthen the tail of a launch that fits the device in one round, as small grids' do)
+    int pmlVector = -1;
+    static constexpr int kPmlTiles = 1024;
+    bool pml_vector() const { return pmlVector < 0 ? nInterior >= kPmlTiles : pmlVector != 0; }
     int rtx = 0, nInterior = 0;   // interior tiles (64*NV x 4 nodes)
     int nPartials = 0, nLevel2 = 0;
     static constexpr int kDirectPartials = 2048;
@@ -345,10 +353,29 @@ template <class R> struct Engine2D {
                         m.si0 = std::min(m.si0, i), m.si1 = std::max(m.si1, i), m.sk0 = std::min(m.sk0, k),
                         m.sk1 = std::max(m.sk1, k);
         }
-        // border nodes (outside the rectangle), compacted: visited tile by tile over 64x4 tiles, row by row
-        // inside each, so consecutive lanes read consecutive nodes of a row
         ntx = (nxc + TX - 1) / TX;
         ntiles = ntx * ((nzc + TZ - 1) / TZ);
+        rtx = i1 > i0 ? (i1 - i0 + 64 * NV - 1) / (64 * NV) : 0;
+        nInterior = rtx ? rtx * ((k1 - k0 + 3) / 4) : 0;
+        build_border_lists();
+    }
+    // The nodes outside the interior rectangle: FP32 elastic CFS-PML models take the ones below the surface band
+    // (depth >= 4) as vector lanes of PML_NV x-adjacent nodes (pml_body; the rectangle's x bounds are multiples of
+    // NV, hence of PML_NV, from column 3, so a lane is wholly inside or outside it), the rest
+    // go to the general list, compacted: visited tile by tile over 64x4 tiles, row by row inside each, so
+    // consecutive lanes read consecutive nodes of a row.
+    void build_border_lists() {
+        const int i0 = m.ri0, i1 = m.ri1, k0 = m.rk0, k1 = m.rk1;
+        const auto inRect = [&](int i, int k) { return i >= i0 && i < i1 && k >= k0 && k < k1; };
+        std::vector<int2> lanes;
+        std::vector<unsigned char> covered((size_t)nx * nz, 0);
+        if (kFast<R> && mech == 0 && cpml && !m.has_damp && pml_vector())
+            for (int k = 7; k < nz - 3; ++k)
+                for (int iv = 3; iv < nx - 3; iv += PML_NV) {
+                    if (inRect(iv, k)) continue;
+                    lanes.push_back(int2{iv, k});
+                    for (int j = 0; j < PML_NV && iv + j < nx - 3; ++j) covered[(size_t)(iv + j) + (size_t)nx * k] = 1;
+                }
         std::vector<int2> border;
         for (int t = 0; t < ntiles; ++t) {
             const int ti0 = 3 + (t % ntx) * TX, ti1 = std::min(nx - 3, ti0 + TX), tk0 = 3 + (t / ntx) * TZ,
@@ -356,17 +383,33 @@ template <class R> struct Engine2D {
             if (ti0 >= i0 && ti1 <= i1 && tk0 >= k0 && tk1 <= k1) continue;
             for (int k = tk0; k < tk1; ++k)
                 for (int i = ti0; i < ti1; ++i)
-                    if (!(i >= i0 && i < i1 && k >= k0 && k < k1)) border.push_back(int2{i, k});
+                    if (!inRect(i, k) && !covered[(size_t)i + (size_t)nx * k]) border.push_back(int2{i, k});
         }
         nBorder = (int)border.size();
         nGeneral = (nBorder + THREADS - 1) / THREADS;
         borderNodes.upload_async(border.empty() ? nullptr : border.data(), std::max<size_t>(border.size(), 1) * sizeof(int2));
-        rtx = i1 > i0 ? (i1 - i0 + 64 * NV - 1) / (64 * NV) : 0;
-        nInterior = rtx ? rtx * ((k1 - k0 + 3) / 4) : 0;
-        nPartials = nGeneral + nInterior;
+        nPml = (int)lanes.size();
+        nPmlBlocks = (nPml + THREADS - 1) / THREADS;
+        pmlLanes.upload_async(lanes.empty() ? nullptr : lanes.data(), std::max<size_t>(lanes.size(), 1) * sizeof(int2));
+        gBlocks = ((nGeneral + 7) / 8 * 8 + nPmlBlocks + 7) / 8 * 8;
+        nPartials = nGeneral + nPmlBlocks + nInterior;
         partials.allocate_async((size_t)std::max(nPartials, 1) * sizeof(Partial));
         nLevel2 = std::max(1, std::min(nPartials, 128));
         level2.allocate_async((size_t)nLevel2 * sizeof(Partial));
+    }
+    void set_pml_vector(int v) {
+        const bool before = pml_vector();
+        const int old = pmlVector;
+        pmlVector = v < 0 ? -1 : v != 0;
+        if (pml_vector() == before) return;
+        if (pending[0].active || pending[1].active) {
+            pmlVector = old;
+            require(false, "pml_vector: a pipelined run is in flight");
+        }
+        DeviceScope scope(device);
+        check(hipStreamSynchronize(stream), "2D border lists"); // the lists and partials are replaced
+        build_border_lists();
+        setup_finish();
     }
 
     // slot = RK stage 0..3; with profiling on, one event pair brackets the four back-to-back stage launches
@@ -382,11 +425,11 @@ template <class R> struct Engine2D {
     template <int DX, int DZ> void launch_both(const StageArgs<R> &a, int slot) {
         const int blocks = a.gblocks + nInterior;
         if (slot == 0) mark(0, true);
-        if (split_border() && !tileLds && a.ngeneral > 0 && nInterior > 0) {
+        if (split_border() && !tileLds && a.gblocks > 0 && nInterior > 0) {
             // border blocks on the side stream beside the interior tiles; the next launch waits for both
             check(hipEventRecord(fork[slot], stream), "2D fork");
             check(hipStreamWaitEvent(side, fork[slot], 0), "2D fork");
-            launch_part<DX, DZ, 1>(a, a.ngeneral, side);
+            launch_part<DX, DZ, 1>(a, a.gblocks, side);
             check(hipGetLastError(), "2D border launch");
             check(hipEventRecord(join[slot], side), "2D join");
             launch_part<DX, DZ, 2>(a, nInterior, stream);
@@ -444,7 +487,9 @@ template <class R> struct Engine2D {
         a.ystage = s == 1;
         a.status = status.as<int>();
         a.partials = partials.as<Partial>();
-        a.border = borderNodes.as<int2>(), a.nborder = nBorder, a.ngeneral = nGeneral, a.gblocks = (nGeneral + 7) / 8 * 8;
+        a.border = borderNodes.as<int2>(), a.nborder = nBorder, a.ngeneral = nGeneral, a.gblocks = gBlocks;
+        a.pml = pmlLanes.as<int2>(), a.npml = nPml, a.pml0 = (nGeneral + 7) / 8 * 8, a.npmlb = nPmlBlocks;
+        a.ipart0 = nGeneral + nPmlBlocks;
         a.ninterior = nInterior, a.rtx = rtx, a.nt = nt, a.spread = spreadBorder;
         return a;
     }
@@ -574,7 +619,7 @@ template <class R> struct Engine2D {
         for (const auto &x : state) b += x.bytes;
         for (const auto &x : pstate) b += x.bytes;
         for (const DeviceBuffer *x : {&accbuf, &paccbuf, &partials, &level2, &probes, &status, &accepted, &peak, &diag,
-                                      &samples, &gather, &display, &borderNodes, &rowTables})
+                                      &samples, &gather, &display, &borderNodes, &pmlLanes, &rowTables})
             b += x->bytes;
         return b;
     }
@@ -856,6 +901,7 @@ int swl_engine2d_kernel_timing(swl_engine2d *e, double *out) {
             out[4] = (double)x.nInterior, out[5] = (double)x.registers;
             out[6] = (double)((x.nx - 6) * (long long)(x.nz - 6)); // nodes updated per stage launch
             out[7] = (double)x.nGeneral, out[8] = (double)x.m.metric, out[9] = (double)x.m.by_row;
+            out[10] = (double)x.nPml, out[11] = (double)x.nPmlBlocks;
         });
     });
 }
@@ -870,6 +916,7 @@ int swl_engine2d_set_option(swl_engine2d *e, const char *name, int value) {
             else if (key == "end_in_stage") x.endInStage = value != 0; // 0: reduce + finalize kernels after every step
             else if (key == "split_border") x.splitBorder = value < 0 ? -1 : value != 0; // -1: automatic
             else if (key == "spread_border") x.spreadBorder = value != 0;
+            else if (key == "pml_vector") x.set_pml_vector(value);
             else throw std::runtime_error("unknown engine option: " + key);
         });
     });
