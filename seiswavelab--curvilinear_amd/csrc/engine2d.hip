@@ -142,15 +142,26 @@ template <class R, int DIR> __device__ inline R coef(int l) {
 }
 template <int DIR> __device__ constexpr int offs(int l) { return DIR ? -(l - 1) : (l - 1); }
 // r / h, computed as r * (1/h) when h is a power of two (the two are then the same IEEE result)
-template <class R> __device__ inline R over_h(R r, R h, R hinv) { return hinv != (R)0 ? r * hinv : r / h; }
+// FP64 multiplies by 1/h only where that is exact (h a power of two) and divides otherwise; FP32 always has hinv.
+// FLAT (FP32 bodies with two or three GSLS mechanisms): multiply unconditionally, which keeps the stencil code one
+// basic block, so that the compiler overlaps several fields' loads (a small grid's launch is bounded by that latency
+// chain); they stay at 3 waves per SIMD.  The elastic bodies keep the branch (at their 5-wave register budget the
+// overlapped loads spill), and so does GSLS-1 (it would drop from 3 to 2 waves).
+template <int MECH> constexpr bool kFlatLoads = MECH >= 2;
+template <class R, bool FLAT = false> __device__ inline R over_h(R r, R h, R hinv) {
+    if constexpr (kFast<R> && FLAT) return r * hinv;
+    else return hinv != (R)0 ? r * hinv : r / h;
+}
 
-template <class R, int DIR> __device__ inline R d5(const R *__restrict__ v, ptrdiff_t q, ptrdiff_t s, R h, R hinv) {
+template <class R, int DIR, bool FLAT = false>
+__device__ inline R d5(const R *__restrict__ v, ptrdiff_t q, ptrdiff_t s, R h, R hinv) {
     R r = 0;
 #pragma unroll
     for (int l = 0; l < 5; ++l) r += coef<R, DIR>(l) * v[q + offs<DIR>(l) * s];
-    return over_h(r, h, hinv);
+    return over_h<R, FLAT>(r, h, hinv);
 }
-template <class R, int DIR> __device__ inline R d3(const R *__restrict__ v, ptrdiff_t q, ptrdiff_t s, R h, R hinv) {
+template <class R, int DIR, bool FLAT = false>
+__device__ inline R d3(const R *__restrict__ v, ptrdiff_t q, ptrdiff_t s, R h, R hinv) {
     const R l0 = ((R)-7) / ((R)6), l1 = ((R)8) / ((R)6), l2 = ((R)-1) / ((R)6);
     const R c0 = DIR ? -l0 : l0, c1 = DIR ? -l1 : l1, c2 = DIR ? -l2 : l2;
     constexpr int o1 = DIR ? -1 : 1, o2 = DIR ? -2 : 2;
@@ -158,15 +169,16 @@ template <class R, int DIR> __device__ inline R d3(const R *__restrict__ v, ptrd
     r += c0 * v[q];
     r += c1 * v[q + o1 * s];
     r += c2 * v[q + o2 * s];
-    return over_h(r, h, hinv);
+    return over_h<R, FLAT>(r, h, hinv);
 }
-template <class R, int DIR> __device__ inline R d2(const R *__restrict__ v, ptrdiff_t q, ptrdiff_t s, R h, R hinv) {
+template <class R, int DIR, bool FLAT = false>
+__device__ inline R d2(const R *__restrict__ v, ptrdiff_t q, ptrdiff_t s, R h, R hinv) {
     const R c0 = DIR ? (R)1 : (R)-1, c1 = DIR ? (R)-1 : (R)1;
     constexpr int o1 = DIR ? -1 : 1;
     R r = 0;
     r += c0 * v[q];
     r += c1 * v[q + o1 * s];
-    return over_h(r, h, hinv);
+    return over_h<R, FLAT>(r, h, hinv);
 }
 
 // ---- compact CFS-PML slots (PointKernels2D.hpp:50-56)
@@ -319,6 +331,7 @@ __device__ inline void rhs_general(const DMesh<R> &m, const R *__restrict__ w, c
                                    int i, int k, R forcing, R *kout, R *prate, long long &sx, long long &sz,
                                    const R (*memIn)[4] = nullptr, R *strain = nullptr) {
     constexpr int F = MECH ? 6 + 4 * MECH : 5;
+    constexpr bool FLAT = kFlatLoads<MECH>;
     const int depth = k - 3;
     const ptrdiff_t q = at(m, i, k), s = m.ld;
     const R h = m.h;
@@ -326,9 +339,9 @@ __device__ inline void rhs_general(const DMesh<R> &m, const R *__restrict__ w, c
     sz = pml_slot(m.nx, m.nz, m.wx, m.wz, m.cpml, i, k, 1);
     R xi[5], eta[5];
 #pragma unroll
-    for (int f = 0; f < 5; ++f) xi[f] = d5<R, DX>(w + (size_t)f * count, q, 1, h, m.hinv);
+    for (int f = 0; f < 5; ++f) xi[f] = d5<R, DX, FLAT>(w + (size_t)f * count, q, 1, h, m.hinv);
 #pragma unroll
-    for (int f = 2; f < 5; ++f) eta[f] = d5<R, DZ>(w + (size_t)f * count, q, s, h, m.hinv);
+    for (int f = 2; f < 5; ++f) eta[f] = d5<R, DZ, FLAT>(w + (size_t)f * count, q, s, h, m.hinv);
     R vx[2], vz[2];
     if (depth == 0) {
         R xs[2];
@@ -341,9 +354,9 @@ __device__ inline void rhs_general(const DMesh<R> &m, const R *__restrict__ w, c
 #pragma unroll
         for (int v = 0; v < 2; ++v) {
             const R *fv = w + (size_t)v * count;
-            eta[v] = depth == 1   ? d2<R, DZ>(fv, q, s, h, m.hinv)
-                     : depth == 2 ? d3<R, DZ>(fv, q, s, h, m.hinv)
-                                  : d5<R, DZ>(fv, q, s, h, m.hinv);
+            eta[v] = depth == 1   ? d2<R, DZ, FLAT>(fv, q, s, h, m.hinv)
+                     : depth == 2 ? d3<R, DZ, FLAT>(fv, q, s, h, m.hinv)
+                                  : d5<R, DZ, FLAT>(fv, q, s, h, m.hinv);
         }
     }
 #pragma unroll
@@ -739,7 +752,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
                     R r = 0;
 #pragma unroll
                     for (int l = 0; l < 5; ++l) r = mad(coef<R, DX>(l), row[T::HV * NV + j + offs<DX>(l)], r);
-                    xi[f][j] = over_h(r, m.h, m.hinv);
+                    xi[f][j] = over_h<R, kFlatLoads<MECH>>(r, m.h, m.hinv);
                 }
                 V col[5];
 #pragma unroll
@@ -749,7 +762,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
                     R r = 0;
 #pragma unroll
                     for (int l = 0; l < 5; ++l) r = mad(coef<R, DZ>(l), col[l][j], r);
-                    eta[f][j] = over_h(r, m.h, m.hinv);
+                    eta[f][j] = over_h<R, kFlatLoads<MECH>>(r, m.h, m.hinv);
                 }
             }
         }
@@ -777,7 +790,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
                 R r = 0;
 #pragma unroll
                 for (int l = 0; l < 5; ++l) r = mad(coef<R, DX>(l), row[HV * NV + j + offs<DX>(l)], r);
-                xi[f][j] = over_h(r, h, m.hinv);
+                xi[f][j] = over_h<R, kFlatLoads<MECH>>(r, h, m.hinv);
             }
             // z: five rows of the same columns
             V col[5];
@@ -788,7 +801,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
                 R r = 0;
 #pragma unroll
                 for (int l = 0; l < 5; ++l) r = mad(coef<R, DZ>(l), col[l][j], r);
-                eta[f][j] = over_h(r, h, m.hinv);
+                eta[f][j] = over_h<R, kFlatLoads<MECH>>(r, h, m.hinv);
             }
         }
         // separable metrics in FP64 (bit-identical to the host arrays; the FP32 path takes one FP64

@@ -71,20 +71,28 @@ class ServeReport:
 
 
 class Server:
-    """budget_bytes: HBM admission budget; chunk: steps between admission / completion checks."""
+    """budget_bytes: HBM admission budget; chunk: steps between admission / completion checks; options: engine
+    switches (Simulation.set_option) applied to every session when it is built; batch_options: runner switches
+    (Batch2D.set_option) of the batch the sessions join."""
 
     def __init__(self, device: int = 0, budget_bytes: float = 200e9, chunk: int = 64, builders: int = 4,
-                 keep_traces: bool = True, largest_first: bool = True):
+                 keep_traces: bool = True, largest_first: bool = True, options: Dict[str, int] = None,
+                 batch_options: Dict[str, int] = None):
         self.device, self.budget, self.chunk, self.builders = device, budget_bytes, chunk, builders
         self.keep_traces = keep_traces
+        self.options = dict(options or {})
+        self.batch_options = dict(batch_options or {})
         # requests admitted together (a backlog) are built largest first, so that the last sessions to join
         # the batch are the quick ones; admission itself stays in arrival order
         self.largest_first = largest_first
 
     @staticmethod
-    def _build(p: Parameters, device: int):
+    def _build(p: Parameters, device: int, options: Dict[str, int]):
         s = Simulation()
         ok, err = s.initialize(p, BackendKind.Hip, device)
+        if ok:
+            for name, value in options.items():
+                s.set_option(name, value)
         return s, ("" if ok else err)
 
     def run(self, requests) -> ServeReport:
@@ -92,6 +100,8 @@ class Server:
         reqs = sorted(list(requests), key=lambda r: r[1])
         results = {i: Result(i, a) for i, a, _ in reqs}
         batch = Batch2D(self.device)
+        for name, value in self.batch_options.items():
+            batch.set_option(name, value)
         pool = cf.ThreadPoolExecutor(max_workers=self.builders)
         queue = list(reqs)  # not yet admitted, in arrival order
         building: Dict[int, cf.Future] = {}
@@ -118,7 +128,7 @@ class Server:
             if self.largest_first:
                 admitted.sort(key=lambda x: -x[0])
             for _, i, p in admitted:
-                building[i] = pool.submit(self._build, p, self.device)
+                building[i] = pool.submit(self._build, p, self.device, self.options)
             # built sessions join the batch between chunks
             for i in [i for i, f in building.items() if f.done()]:
                 s, err = building.pop(i).result()
