@@ -208,10 +208,10 @@ int swl_engine2d_run_collect(swl_engine2d *e, int slot, double *diag, double *sa
    back-to-back stage launches of a step; the finalisation is timed on every 32nd step.
    enable resets the counters. */
 int swl_engine2d_profile(swl_engine2d *e, int enable);
-/* out[12]: stage-kernel ms, stage launches, finalisation ms, finalisation launches, interior tiles,
+/* out[13]: stage-kernel ms, stage launches, finalisation ms, finalisation launches, interior tiles,
    stage-kernel VGPRs, computational nodes per stage launch, border tiles (general blocks), on-the-fly metrics
    (0/1), per-row material tables (1: lambda, mu, rho, dP, dS depend on the row only; 0: per-node arrays),
-   PML vector lanes (two x-adjacent nodes each), PML blocks. */
+   PML vector lanes (two x-adjacent nodes each), PML blocks, small-grid elastic kernel in use (0/1). */
 int swl_engine2d_kernel_timing(swl_engine2d *e, double *out);
 /* Engine tuning switches: "nontemporal_stores" (default 1) for the interior outputs;
    "stored_metrics" (default 0) reads a, b, J from memory instead of recomputing them;
@@ -221,6 +221,9 @@ int swl_engine2d_kernel_timing(swl_engine2d *e, double *out);
    "pml_vector" (-1 automatic: grids of at least 1024 interior tiles; 0 off; 1 on): FP32 elastic CFS-PML models
    update the absorbing strips below the surface band (depth >= 4, no sponge) as lanes of two x-adjacent nodes
    instead of one node per lane (same equations, rounded differently: FP32 tolerance, not bit-identical).
+   "latency_kernel" (-1 automatic: FP32 elastic launches of fewer than 1536 blocks; 0 off; 1 on): a variant of the
+   stage kernel that keeps several fields' stencil loads in flight at a 3-wave register budget, for grids too small
+   to fill the device (identical results).
    "end_in_stage" (default 1): step j is ended (reduction, fault rule, diagnostics and receiver rows) by block 0
    of step j + 1's first stage launch; 0 launches a reduction and a finalisation kernel after every step.
    "split_border" (-1 automatic: large FP64 grids; 0 off; 1 on): the border blocks run as their own launch on the

@@ -143,11 +143,12 @@ template <class R, int DIR> __device__ inline R coef(int l) {
 template <int DIR> __device__ constexpr int offs(int l) { return DIR ? -(l - 1) : (l - 1); }
 // r / h, computed as r * (1/h) when h is a power of two (the two are then the same IEEE result)
 // FP64 multiplies by 1/h only where that is exact (h a power of two) and divides otherwise; FP32 always has hinv.
-// FLAT (FP32 bodies with two or three GSLS mechanisms): multiply unconditionally, which keeps the stencil code one
-// basic block, so that the compiler overlaps several fields' loads (a small grid's launch is bounded by that latency
-// chain); they stay at 3 waves per SIMD.  The elastic bodies keep the branch (at their 5-wave register budget the
-// overlapped loads spill), and so does GSLS-1 (it would drop from 3 to 2 waves).
-template <int MECH> constexpr bool kFlatLoads = MECH >= 2;
+// FLAT (FP32 bodies with two or three GSLS mechanisms, and the small-grid elastic kernel): multiply
+// unconditionally, which keeps the stencil code one basic block, so that the compiler overlaps several fields' loads
+// (a small grid's launch is bounded by that latency chain); they stay at 3 waves per SIMD.  The elastic bodies of
+// the large-grid kernel keep the branch (at their 5-wave register budget the overlapped loads spill), and so does
+// GSLS-1 (it would drop from 3 to 2 waves).
+template <int MECH, bool LAT = false> constexpr bool kFlatLoads = LAT || MECH >= 2;
 template <class R, bool FLAT = false> __device__ inline R over_h(R r, R h, R hinv) {
     if constexpr (kFast<R> && FLAT) return r * hinv;
     else return hinv != (R)0 ? r * hinv : r / h;
@@ -326,12 +327,12 @@ __device__ inline void closure(const DMesh<R> &m, const R *__restrict__ w, size_
 // Full RHS of one node outside the interior rectangle (PointKernels2D.hpp:121-172).  REBUILD: the memory
 // variables' stage inputs come from memIn (the rates of the memories, Syy and their sponge terms are then
 // left to the caller) and the node's strains exx, ezz, exz go to strain.
-template <class R, int DX, int DZ, int MECH, bool REBUILD = false>
+template <class R, int DX, int DZ, int MECH, bool REBUILD = false, bool LAT = false>
 __device__ inline void rhs_general(const DMesh<R> &m, const R *__restrict__ w, const R *__restrict__ pin, size_t count,
                                    int i, int k, R forcing, R *kout, R *prate, long long &sx, long long &sz,
                                    const R (*memIn)[4] = nullptr, R *strain = nullptr) {
     constexpr int F = MECH ? 6 + 4 * MECH : 5;
-    constexpr bool FLAT = kFlatLoads<MECH>;
+    constexpr bool FLAT = kFlatLoads<MECH, LAT>;
     const int depth = k - 3;
     const ptrdiff_t q = at(m, i, k), s = m.ld;
     const R h = m.h;
@@ -503,7 +504,7 @@ __device__ inline void node_stats(R rho, R lam, R mu, R jac, R h, const R *u, bo
 // ------------------------------------------------------------------------------------------------
 // General kernel: block li takes border nodes [256 li, 256 li + 256) of the compacted list (every lane
 // holds a node; the lists follow the rows of 64x4 tiles, so a wave still reads row segments).
-template <class R, int DX, int DZ, int MECH>
+template <class R, int DX, int DZ, int MECH, bool LAT = false>
 __device__ __forceinline__ void general_body(const StageArgs<R> &A, int li, Partial *sh) {
     constexpr int F = MECH ? 6 + 4 * MECH : 5;
     const int n = li * THREADS + (int)threadIdx.x;
@@ -542,9 +543,9 @@ __device__ __forceinline__ void general_body(const StageArgs<R> &A, int li, Part
                 }
                 memIn[l][3] = 0; // the yy memories enter Syy only, which is formed at the last stage
             }
-            rhs_general<R, DX, DZ, MECH, true>(m, A.in, A.pin, cnt, i, k, A.forcing, kv, pr, sx, sz, memIn, strain);
+            rhs_general<R, DX, DZ, MECH, true, LAT>(m, A.in, A.pin, cnt, i, k, A.forcing, kv, pr, sx, sz, memIn, strain);
         } else {
-            rhs_general<R, DX, DZ, MECH>(m, A.in, A.pin, cnt, i, k, A.forcing, kv, pr, sx, sz);
+            rhs_general<R, DX, DZ, MECH, false, LAT>(m, A.in, A.pin, cnt, i, k, A.forcing, kv, pr, sx, sz);
         }
 #pragma unroll
         for (int f = 0; f < FW; ++f) {
@@ -708,7 +709,7 @@ template <class R> struct Tile {
     using V = typename Lane<R>::V;
     V v[2][ROWS][W];
 };
-template <class R, int DX, int DZ, int MECH, bool TILE = false>
+template <class R, int DX, int DZ, int MECH, bool TILE = false, bool LAT = false>
 __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Partial *sh, Tile<R> *tile = nullptr) {
     using V = typename Lane<R>::V;
     constexpr int NV = Lane<R>::N;
@@ -752,7 +753,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
                     R r = 0;
 #pragma unroll
                     for (int l = 0; l < 5; ++l) r = mad(coef<R, DX>(l), row[T::HV * NV + j + offs<DX>(l)], r);
-                    xi[f][j] = over_h<R, kFlatLoads<MECH>>(r, m.h, m.hinv);
+                    xi[f][j] = over_h<R, kFlatLoads<MECH, LAT>>(r, m.h, m.hinv);
                 }
                 V col[5];
 #pragma unroll
@@ -762,7 +763,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
                     R r = 0;
 #pragma unroll
                     for (int l = 0; l < 5; ++l) r = mad(coef<R, DZ>(l), col[l][j], r);
-                    eta[f][j] = over_h<R, kFlatLoads<MECH>>(r, m.h, m.hinv);
+                    eta[f][j] = over_h<R, kFlatLoads<MECH, LAT>>(r, m.h, m.hinv);
                 }
             }
         }
@@ -790,7 +791,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
                 R r = 0;
 #pragma unroll
                 for (int l = 0; l < 5; ++l) r = mad(coef<R, DX>(l), row[HV * NV + j + offs<DX>(l)], r);
-                xi[f][j] = over_h<R, kFlatLoads<MECH>>(r, h, m.hinv);
+                xi[f][j] = over_h<R, kFlatLoads<MECH, LAT>>(r, h, m.hinv);
             }
             // z: five rows of the same columns
             V col[5];
@@ -801,7 +802,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
                 R r = 0;
 #pragma unroll
                 for (int l = 0; l < 5; ++l) r = mad(coef<R, DZ>(l), col[l][j], r);
-                eta[f][j] = over_h<R, kFlatLoads<MECH>>(r, h, m.hinv);
+                eta[f][j] = over_h<R, kFlatLoads<MECH, LAT>>(r, h, m.hinv);
             }
         }
         // separable metrics in FP64 (bit-identical to the host arrays; the FP32 path takes one FP64
@@ -1057,7 +1058,8 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
 // a multiple of the 8 XCDs, so the interior blocks keep their XCD-contiguous band mapping), the rest
 // take the interior tiles.  A single in-order launch avoids cross-stream joins between stages.
 // The FP32 elastic stage fits 96 VGPRs without spilling, which gives 5 waves per SIMD instead of 4.
-template <class R, int MECH> constexpr int kWavesPerEU = sizeof(R) == 4 && MECH == 0 ? 5 : 1;
+// LAT: the small-grid kernel (a few waves per SIMD at most, so the register budget of 3 waves costs nothing).
+template <class R, int MECH, bool LAT = false> constexpr int kWavesPerEU = LAT ? 3 : sizeof(R) == 4 && MECH == 0 ? 5 : 1;
 
 // The reference's fault rule and peak update for step `step` on the reduced inspection r
 // (CurvilinearSolver2D.cpp:576-590): not finite, or after 3/f an energy above 1e4 x the accepted peak, faults the
@@ -1299,8 +1301,9 @@ template <class R> __device__ inline void block_role(const StageArgs<R> &A, int 
 
 // PART: 0 one launch for the whole grid (border blocks and interior tiles, see block_role); 1 the border blocks
 // only; 2 the interior tiles only.  Parts 1 and 2 run as two concurrent launches, each with its own register budget.
-template <class R, int DX, int DZ, int MECH, bool TILE = false, int PART = 0>
-__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<R, MECH>))) void stage_kernel(
+// LAT: the small-grid variant (flat stencil loads at a 3-wave register budget; same arithmetic, same results).
+template <class R, int DX, int DZ, int MECH, bool TILE = false, int PART = 0, bool LAT = false>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<R, MECH, LAT>))) void stage_kernel(
     StageArgs<R> A) {
     __shared__ Partial sh[THREADS];
     if (*A.status) return;
@@ -1319,7 +1322,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(kWavesP
     }
     if (border >= 0) {
         if (border < A.ngeneral) {
-            general_body<R, DX, DZ, MECH>(A, border, sh);
+            general_body<R, DX, DZ, MECH, LAT>(A, border, sh);
         } else if constexpr (kFast<R> && MECH == 0) {
             if (border >= A.pml0 && border < A.pml0 + A.npmlb) pml_body<R, DX, DZ>(A, border - A.pml0, sh);
         }
@@ -1330,7 +1333,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(kWavesP
             __shared__ Tile<R> tile;
             interior_body<R, DX, DZ, MECH, true>(A, tile_of_block(interior, A.ninterior), sh, &tile);
         } else {
-            interior_body<R, DX, DZ, MECH>(A, tile_of_block(interior, A.ninterior), sh);
+            interior_body<R, DX, DZ, MECH, false, LAT>(A, tile_of_block(interior, A.ninterior), sh);
         }
     }
 }

@@ -82,6 +82,14 @@ template <class R> struct Engine2D {
     // longer border threads lengthen the tail of a launch that fits the device in one round, as small grids' do)
     int pmlVector = -1;
     static constexpr int kPmlTiles = 1024;
+    // small-grid FP32 elastic kernel (option "latency_kernel"): -1 automatic (launches of fewer than kLatBlocks
+    // blocks, a few waves per SIMD at most), 0 off, 1 on; same arithmetic, same results
+    int latencyKernel = -1;
+    static constexpr int kLatBlocks = 1536;
+    bool latency_kernel() const {
+        if (!kFast<R> || mech != 0) return false;
+        return latencyKernel < 0 ? gBlocks + nInterior < kLatBlocks : latencyKernel != 0;
+    }
     bool pml_vector() const { return pmlVector < 0 ? nInterior >= kPmlTiles : pmlVector != 0; }
     int rtx = 0, nInterior = 0;   // interior tiles (64*NV x 4 nodes)
     int nPartials = 0, nLevel2 = 0;
@@ -441,7 +449,9 @@ template <class R> struct Engine2D {
         switch (mech) {
         case 0:
             if (tileLds) stage_kernel<R, DX, DZ, 0, true><<<blocks, THREADS, 0, stream>>>(a);
-            else stage_kernel<R, DX, DZ, 0><<<blocks, THREADS, 0, stream>>>(a);
+            else if (kFast<R> && latency_kernel()) {
+                if constexpr (kFast<R>) stage_kernel<R, DX, DZ, 0, false, 0, true><<<blocks, THREADS, 0, stream>>>(a);
+            } else stage_kernel<R, DX, DZ, 0><<<blocks, THREADS, 0, stream>>>(a);
             break;
         case 1: stage_kernel<R, DX, DZ, 1><<<blocks, THREADS, 0, stream>>>(a); break;
         case 2: stage_kernel<R, DX, DZ, 2><<<blocks, THREADS, 0, stream>>>(a); break;
@@ -901,7 +911,7 @@ int swl_engine2d_kernel_timing(swl_engine2d *e, double *out) {
             out[4] = (double)x.nInterior, out[5] = (double)x.registers;
             out[6] = (double)((x.nx - 6) * (long long)(x.nz - 6)); // nodes updated per stage launch
             out[7] = (double)x.nGeneral, out[8] = (double)x.m.metric, out[9] = (double)x.m.by_row;
-            out[10] = (double)x.nPml, out[11] = (double)x.nPmlBlocks;
+            out[10] = (double)x.nPml, out[11] = (double)x.nPmlBlocks, out[12] = (double)x.latency_kernel();
         });
     });
 }
@@ -917,6 +927,7 @@ int swl_engine2d_set_option(swl_engine2d *e, const char *name, int value) {
             else if (key == "split_border") x.splitBorder = value < 0 ? -1 : value != 0; // -1: automatic
             else if (key == "spread_border") x.spreadBorder = value != 0;
             else if (key == "pml_vector") x.set_pml_vector(value);
+            else if (key == "latency_kernel") x.latencyKernel = value < 0 ? -1 : value != 0;
             else throw std::runtime_error("unknown engine option: " + key);
         });
     });

@@ -195,12 +195,12 @@ class Simulation:
 
     def kernel_timing(self) -> dict:
         self._need()
-        o = np.zeros(12)
+        o = np.zeros(13)
         native.call("swl_sim2d_kernel_timing", self._h, _dp(o))
         return {"stage_ms": o[0], "stage_launches": int(o[1]), "finalize_ms": o[2], "finalize_launches": int(o[3]),
                 "interior_tiles": int(o[4]), "vgprs": int(o[5]), "nodes_per_launch": int(o[6]),
                 "border_tiles": int(o[7]), "metric_on_the_fly": bool(o[8]), "material_by_row": bool(o[9]),
-                "pml_lanes": int(o[10]), "pml_blocks": int(o[11])}
+                "pml_lanes": int(o[10]), "pml_blocks": int(o[11]), "latency_kernel": bool(o[12])}
 
     def fault(self) -> str:
         """The latched fault message ("" if none)."""
@@ -215,7 +215,7 @@ class Simulation:
 
     def set_option(self, name: str, value: int):
         """Engine tuning switch (swl_engine2d_set_option): "nontemporal_stores", "stored_metrics", "lds_tiles",
-        "material_rows", "pml_vector", "end_in_stage", "split_border", "spread_border"."""
+        "material_rows", "pml_vector", "latency_kernel", "end_in_stage", "split_border", "spread_border"."""
         self._need()
         native.call("swl_sim2d_set_option", self._h, name.encode(), int(value))
 

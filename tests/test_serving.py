@@ -69,3 +69,21 @@ def test_sequential_fp64_path_matches_fp32_within_tolerance():
         ga = np.concatenate([a.traces["vx"].ravel(), a.traces["vz"].ravel()])
         gb = np.concatenate([b.traces["vx"].ravel(), b.traces["vz"].ravel()])
         assert np.sqrt(np.sum((ga - gb) ** 2) / np.sum(gb * gb)) <= 1e-4
+
+
+def test_session_and_batch_options_reach_the_server_path():
+    """Server(options=..., batch_options=...): engine switches that change only the schedule (no nontemporal
+    stores, two lanes of 5 steps) leave every trace equal to the request's lone run; a switch that changes FP32
+    rounding (PML lanes forced on) keeps it within the FP32 bar of the lone run."""
+    reqs = small_requests(n=4)
+    same = Server(chunk=16, options={"nontemporal_stores": 0},
+                  batch_options={"lanes": 2, "lane_steps": 5}).run(copy.deepcopy(reqs))
+    pml = Server(chunk=16, options={"pml_vector": 1}).run(copy.deepcopy(reqs))
+    assert same.stats["lanes"] == 2
+    for r, q, (i, _, p) in zip(same.results, pml.results, reqs):
+        ref = alone(p)
+        assert not r.error and not q.error
+        assert np.array_equal(r.traces["vx"], ref["vx"]) and np.array_equal(r.traces["vz"], ref["vz"]), i
+        ga = np.concatenate([q.traces["vx"].ravel(), q.traces["vz"].ravel()])
+        gb = np.concatenate([ref["vx"].ravel(), ref["vz"].ravel()])
+        assert np.sqrt(np.sum((ga - gb) ** 2) / np.sum(gb * gb)) <= 2e-5, i

@@ -124,13 +124,30 @@ def test_fault_inside_pipelined_runs(prec):
 @pytest.mark.parametrize("gsls", [False, True])
 def test_launch_layouts_equal(prec, gsls):
     """The border blocks launched on the engine's second stream beside the interior tiles (split_border, automatic
-    for large FP64 grids) or spread evenly over the one launch (spread_border) change only the schedule: every
-    field, receiver sample and the energy are the same to the bit."""
+    for large FP64 grids), spread evenly over the one launch (spread_border), or the small-grid FP32 elastic kernel
+    forced on this large grid (latency_kernel; it changes only the code's schedule) leave every field, receiver
+    sample and the energy the same to the bit."""
     p = grf_case(prec, gsls)
     chunks = [40, 50]
     print(f"[launch layouts] {prec.name} gsls={gsls}", flush=True)
     base = run(p, {"split_border": 0, "spread_border": 0}, chunks)
-    for opts in ({"split_border": 1}, {"spread_border": 1}):
+    for opts in ({"split_border": 1}, {"spread_border": 1}, {"latency_kernel": 1}):
         other = run(p, opts, chunks)
         assert_same(other, base)
         assert other["diag"]["energy"] == base["diag"]["energy"], opts
+        assert other["timing"]["latency_kernel"] == (opts.get("latency_kernel") == 1 and not gsls
+                                                     and prec == Precision.Float32)
+    assert not base["timing"]["latency_kernel"]  # 2400 x 900: more blocks than the automatic choice takes
+
+
+def test_latency_kernel_automatic_on_small_grids():
+    """A small FP32 elastic grid takes the small-grid kernel by itself; forced off it gives the same run, bit for
+    bit (the two kernels differ only in how their loads are scheduled)."""
+    p = ref_case("moment_external_pml")
+    p.precision = Precision.Float32
+    chunks = [37, 1, 52]
+    auto = run(p, {}, chunks)
+    off = run(p, {"latency_kernel": 0}, chunks)
+    assert auto["timing"]["latency_kernel"] and not off["timing"]["latency_kernel"]
+    assert_same(auto, off)
+    assert auto["diag"]["energy"] == off["diag"]["energy"]
