@@ -38,7 +38,7 @@ constexpr int ORIGIN = 61; // off: column 3 lands on a 64-element boundary
 template <class R> struct DMesh3 {
     int nx, ny, nz, ld, off, flat;
     long long plane; // ld * ny
-    R h, hinv;       // hinv = 1/h when h is a power of two (exact), else 0
+    R h, hinv;       // hinv = 1/h: FP32 always, FP64 when h is a power of two (exact), else 0 (divide)
     const R *met;    // 9 arrays of mcount elements (k < flat)
     const R *jac;    // mcount (k < flat)
     long long mcount;

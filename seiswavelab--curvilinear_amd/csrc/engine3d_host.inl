@@ -90,7 +90,7 @@ template <class R> struct Engine3D {
         m.nx = nx, m.ny = ny, m.nz = nz, m.ld = ld, m.off = off, m.plane = (long long)ld * ny;
         m.h = (R)d.h;
         int e2 = 0;
-        m.hinv = std::frexp(d.h, &e2) == 0.5 ? (R)(1 / d.h) : (R)0;
+        m.hinv = sizeof(R) == 4 || std::frexp(d.h, &e2) == 0.5 ? (R)(1 / d.h) : (R)0;
         // metric band: identity and J = 1 for k >= flat (checked)
         int flat = std::clamp(d.flat_from, 3, nz);
         for (int k = flat; k < nz - 3 && flat < nz; ++k)
