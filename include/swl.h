@@ -208,14 +208,17 @@ int swl_engine2d_run_collect(swl_engine2d *e, int slot, double *diag, double *sa
    back-to-back stage launches of a step; the finalisation is timed on every 32nd step.
    enable resets the counters. */
 int swl_engine2d_profile(swl_engine2d *e, int enable);
-/* out[13]: stage-kernel ms, stage launches, finalisation ms, finalisation launches, interior tiles,
+/* out[14]: stage-kernel ms, stage launches, finalisation ms, finalisation launches, interior tiles,
    stage-kernel VGPRs, computational nodes per stage launch, border tiles (general blocks), on-the-fly metrics
    (0/1), per-row material tables (1: lambda, mu, rho, dP, dS depend on the row only; 0: per-node arrays),
-   PML vector lanes (two x-adjacent nodes each), PML blocks, small-grid elastic kernel in use (0/1). */
+   PML vector lanes (two x-adjacent nodes each), PML blocks, small-grid elastic kernel in use (0/1), LDS tiles in
+   use (0/1). */
 int swl_engine2d_kernel_timing(swl_engine2d *e, double *out);
 /* Engine tuning switches: "nontemporal_stores" (default 1) for the interior outputs;
    "stored_metrics" (default 0) reads a, b, J from memory instead of recomputing them;
-   "lds_tiles" (default 0) stages the elastic interior's stencil input through LDS;
+   "lds_tiles" (-1 automatic: elastic launches of at least 1536 blocks; 0 off; 1 on) loads each field's tile of
+   the elastic interior's stencil input straight into LDS (global_load_lds), one field ahead of the arithmetic
+   (identical results);
    "material_rows" (default 1 where the materials depend on the row only) reads the per-row tables,
    0 reads the per-node arrays;
    "pml_vector" (-1 automatic: grids of at least 1024 interior tiles; 0 off; 1 on): FP32 elastic CFS-PML models

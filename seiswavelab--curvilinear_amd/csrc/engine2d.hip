@@ -709,6 +709,38 @@ template <class R> struct Tile {
     using V = typename Lane<R>::V;
     V v[2][ROWS][W];
 };
+// One field's stencil taps from an LDS tile: the 2 HV + 1 vectors of the lane's row (x) and the four other rows of
+// its column (z), read by one block of ds_read_b128 that waits for them itself.  The reads are written out so that
+// the compiler, which cannot tell the tile's two buffers apart, does not wait for the other buffer's loads (still
+// in flight by design) before them.  base: LDS address of the wave's first staged row at the lane's first vector;
+// the immediates are byte offsets from it (x taps in row X0, z taps in rows Z[0..3], column HV).  The outputs are
+// early-clobber: each read is issued while base is still needed by the next.
+template <class R, int X0, int Z0, int Z1, int Z2, int Z3>
+__device__ __forceinline__ void lds_taps(unsigned base, typename Lane<R>::V (&x)[2 * Tile<R>::HV + 1],
+                                         typename Lane<R>::V (&z)[4]) {
+    using T = Tile<R>;
+    constexpr int S = 16, RW = T::W * S, C = T::HV * S;
+    if constexpr (T::HV == 1) {
+        asm volatile("ds_read_b128 %0, %7 offset:%8\n\tds_read_b128 %1, %7 offset:%9\n\tds_read_b128 %2, %7 offset:%10\n\t"
+                     "ds_read_b128 %3, %7 offset:%11\n\tds_read_b128 %4, %7 offset:%12\n\tds_read_b128 %5, %7 offset:%13\n\t"
+                     "ds_read_b128 %6, %7 offset:%14\n\ts_waitcnt lgkmcnt(0)"
+                     : "=&v"(x[0]), "=&v"(x[1]), "=&v"(x[2]), "=&v"(z[0]), "=&v"(z[1]), "=&v"(z[2]), "=&v"(z[3])
+                     : "v"(base), "i"(X0 * RW), "i"(X0 * RW + S), "i"(X0 * RW + 2 * S), "i"(Z0 * RW + C),
+                       "i"(Z1 * RW + C), "i"(Z2 * RW + C), "i"(Z3 * RW + C)
+                     : "memory");
+    } else {
+        static_assert(T::HV == 2, "two or four nodes per lane");
+        asm volatile("ds_read_b128 %0, %9 offset:%10\n\tds_read_b128 %1, %9 offset:%11\n\tds_read_b128 %2, %9 offset:%12\n\t"
+                     "ds_read_b128 %3, %9 offset:%13\n\tds_read_b128 %4, %9 offset:%14\n\tds_read_b128 %5, %9 offset:%15\n\t"
+                     "ds_read_b128 %6, %9 offset:%16\n\tds_read_b128 %7, %9 offset:%17\n\tds_read_b128 %8, %9 offset:%18\n\t"
+                     "s_waitcnt lgkmcnt(0)"
+                     : "=&v"(x[0]), "=&v"(x[1]), "=&v"(x[2]), "=&v"(x[3]), "=&v"(x[4]), "=&v"(z[0]), "=&v"(z[1]),
+                       "=&v"(z[2]), "=&v"(z[3])
+                     : "v"(base), "i"(X0 * RW), "i"(X0 * RW + S), "i"(X0 * RW + 2 * S), "i"(X0 * RW + 3 * S),
+                       "i"(X0 * RW + 4 * S), "i"(Z0 * RW + C), "i"(Z1 * RW + C), "i"(Z2 * RW + C), "i"(Z3 * RW + C)
+                     : "memory");
+    }
+}
 template <class R, int DX, int DZ, int MECH, bool TILE = false, bool LAT = false>
 __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Partial *sh, Tile<R> *tile = nullptr) {
     using V = typename Lane<R>::V;
@@ -726,24 +758,56 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
         using T = Tile<R>;
         constexpr int rlo = DZ ? -3 : -1; // first staged row relative to bk
         const size_t cnt = A.count;
+        // Field f's tile (rows bk + rlo .. bk + rlo + 7, vectors bx - HV NV .. bx + (64 + HV) NV) goes straight from
+        // global memory into LDS buffer f & 1 (global_load_lds, 16 bytes per lane; lane L of an instruction writes
+        // slot L after the wave-uniform LDS address).  Wave w loads rows 2w and 2w + 1: one instruction per row for
+        // the 64 lane vectors, one for each side's HV halo vectors (lanes 0 .. HV - 1 only).  Every wave issues
+        // exactly DMA_PER_FIELD instructions per field, whatever the grid edge: addresses of slots that only dead
+        // lanes read are clamped into the row instead of skipped, so the counter arithmetic below is exact.
+        constexpr int DMA_PER_FIELD = 6;
+        const auto dma = [&](const R *g, V *l) {
+            __builtin_amdgcn_global_load_lds(static_cast<const void *>(g), (__attribute__((address_space(3))) void *)l, 16, 0, 0);
+        };
+        const auto clamp_row = [&](int kr) { return kr < m.nz ? kr : m.nz - 1; };
+        const auto clamp_x = [&](int ix) { return ix < -m.off ? -m.off : ix + NV > m.ld - m.off ? m.ld - m.off - NV : ix; };
+        const auto issue = [&](int f) {
+            const R *fp = A.in + (size_t)f * cnt;
+            auto &buf = tile->v[f & 1];
+#pragma unroll
+            for (int rr = 0; rr < 2; ++rr) {
+                const int r = 2 * wrow + rr, kr = clamp_row(bk + rlo + r);
+                dma(fp + at(m, clamp_x(bx + lane * NV), kr), &buf[r][T::HV]);
+                if (lane < T::HV) {
+                    dma(fp + at(m, clamp_x(bx + (lane - T::HV) * NV), kr), &buf[r][0]);
+                    dma(fp + at(m, clamp_x(bx + (64 + lane) * NV), kr), &buf[r][T::HV + 64]);
+                }
+            }
+        };
+        // vmcnt / expcnt / lgkmcnt fields of s_waitcnt (gfx9 encoding); the unused counters at their maximum
+        constexpr int kWaitNextField = (DMA_PER_FIELD & 15) | ((DMA_PER_FIELD >> 4) << 14) | (7 << 4) | (15 << 8);
+        constexpr int kWaitAllLoads = (7 << 4) | (15 << 8);
+        constexpr int kWaitLds = 15 | (3 << 14) | (7 << 4); // lgkmcnt(0)
+        issue(0);
 #pragma unroll
         for (int f = 0; f < 5; ++f) {
             auto &buf = tile->v[f & 1];
-            // rows bk + rlo .. bk + rlo + 7 of vectors bx - HV NV .. bx + (64 + HV) NV; nothing outside the
-            // allocation is read (rows < nz, vectors inside the row's ld span; the rest are only dead lanes')
-            for (int e = threadIdx.x; e < T::ROWS * T::W; e += THREADS) {
-                const int r = e / T::W, v = e % T::W, kr = bk + rlo + r, ix = bx + (v - T::HV) * NV;
-                V x{};
-                if (kr < m.nz && m.off + ix >= 0 && m.off + ix + NV <= m.ld) x = vload(A.in + (size_t)f * cnt + at(m, ix, kr));
-                buf[r][v] = x;
+            if (f + 1 < 5) {
+                issue(f + 1); // into the other buffer, released by the barrier that ended field f - 1
+                __builtin_amdgcn_s_waitcnt(kWaitNextField); // this wave's field-f loads have landed
+            } else {
+                __builtin_amdgcn_s_waitcnt(kWaitAllLoads);
             }
-            __syncthreads();
+            __builtin_amdgcn_s_barrier(); // ... and every wave's
             if (live) {
-                const int rr = wrow - rlo;
+                // rows relative to the wave's first staged row (wrow): the lane's own row is -rlo, tap l's row
+                // offs(l) - rlo (l = 1 is the centre, in the x taps)
+                V xt[2 * T::HV + 1], zt[4];
+                const unsigned base = (unsigned)(uintptr_t)(__attribute__((address_space(3))) V *)&buf[wrow][lane];
+                lds_taps<R, -rlo, offs<DZ>(0) - rlo, offs<DZ>(2) - rlo, offs<DZ>(3) - rlo, offs<DZ>(4) - rlo>(base, xt, zt);
                 R row[(2 * T::HV + 1) * NV];
 #pragma unroll
                 for (int v = 0; v <= 2 * T::HV; ++v) {
-                    const V x = buf[rr][lane + v];
+                    const V x = xt[v];
                     if (v == T::HV) centre[f] = x;
 #pragma unroll
                     for (int j = 0; j < NV; ++j) row[v * NV + j] = x[j];
@@ -755,9 +819,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
                     for (int l = 0; l < 5; ++l) r = mad(coef<R, DX>(l), row[T::HV * NV + j + offs<DX>(l)], r);
                     xi[f][j] = over_h<R, kFlatLoads<MECH, LAT>>(r, m.h, m.hinv);
                 }
-                V col[5];
-#pragma unroll
-                for (int l = 0; l < 5; ++l) col[l] = buf[rr + offs<DZ>(l)][lane + T::HV];
+                const V col[5] = {zt[0], centre[f], zt[1], zt[2], zt[3]};
 #pragma unroll
                 for (int j = 0; j < NV; ++j) {
                     R r = 0;
@@ -766,6 +828,8 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
                     eta[f][j] = over_h<R, kFlatLoads<MECH, LAT>>(r, m.h, m.hinv);
                 }
             }
+            __builtin_amdgcn_s_waitcnt(kWaitLds); // this wave's reads of buffer f & 1 have returned
+            __builtin_amdgcn_s_barrier();         // before any wave refills it with field f + 2
         }
     }
     if (live) {

@@ -195,12 +195,13 @@ class Simulation:
 
     def kernel_timing(self) -> dict:
         self._need()
-        o = np.zeros(13)
+        o = np.zeros(14)
         native.call("swl_sim2d_kernel_timing", self._h, _dp(o))
         return {"stage_ms": o[0], "stage_launches": int(o[1]), "finalize_ms": o[2], "finalize_launches": int(o[3]),
                 "interior_tiles": int(o[4]), "vgprs": int(o[5]), "nodes_per_launch": int(o[6]),
                 "border_tiles": int(o[7]), "metric_on_the_fly": bool(o[8]), "material_by_row": bool(o[9]),
-                "pml_lanes": int(o[10]), "pml_blocks": int(o[11]), "latency_kernel": bool(o[12])}
+                "pml_lanes": int(o[10]), "pml_blocks": int(o[11]), "latency_kernel": bool(o[12]),
+                "lds_tiles": bool(o[13])}
 
     def fault(self) -> str:
         """The latched fault message ("" if none)."""

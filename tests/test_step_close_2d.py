@@ -123,21 +123,23 @@ def test_fault_inside_pipelined_runs(prec):
 @pytest.mark.parametrize("prec", [Precision.Float64, Precision.Float32])
 @pytest.mark.parametrize("gsls", [False, True])
 def test_launch_layouts_equal(prec, gsls):
-    """The border blocks launched on the engine's second stream beside the interior tiles (split_border, automatic
-    for large FP64 grids), spread evenly over the one launch (spread_border), or the small-grid FP32 elastic kernel
-    forced on this large grid (latency_kernel; it changes only the code's schedule) leave every field, receiver
-    sample and the energy the same to the bit."""
+    """The border blocks launched on the engine's second stream beside the interior tiles (split_border), spread
+    evenly over the one launch (spread_border), the small-grid FP32 elastic kernel forced on this large grid
+    (latency_kernel), or the elastic interior's direct loads instead of its LDS tiles (lds_tiles 0; the tiles are
+    automatic at this size) change only the schedule: every field, receiver sample and the energy are the same to
+    the bit."""
     p = grf_case(prec, gsls)
     chunks = [40, 50]
     print(f"[launch layouts] {prec.name} gsls={gsls}", flush=True)
     base = run(p, {"split_border": 0, "spread_border": 0}, chunks)
-    for opts in ({"split_border": 1}, {"spread_border": 1}, {"latency_kernel": 1}):
+    for opts in ({"split_border": 1}, {"spread_border": 1}, {"latency_kernel": 1}, {"lds_tiles": 0}):
         other = run(p, opts, chunks)
         assert_same(other, base)
         assert other["diag"]["energy"] == base["diag"]["energy"], opts
         assert other["timing"]["latency_kernel"] == (opts.get("latency_kernel") == 1 and not gsls
                                                      and prec == Precision.Float32)
     assert not base["timing"]["latency_kernel"]  # 2400 x 900: more blocks than the automatic choice takes
+    assert base["timing"]["lds_tiles"] == (not gsls)  # elastic: the interior's input staged through LDS
 
 
 def test_latency_kernel_automatic_on_small_grids():
