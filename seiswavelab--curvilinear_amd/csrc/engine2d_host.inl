@@ -431,7 +431,11 @@ template <class R> struct Engine2D {
     // (an event between launches would itself add a dispatch gap)
     template <int DX, int DZ, int PART> void launch_part(const StageArgs<R> &a, int blocks, hipStream_t s) {
         switch (mech) {
-        case 0: stage_kernel<R, DX, DZ, 0, false, PART><<<blocks, THREADS, 0, s>>>(a); break;
+        case 0:
+            // the interior-only launch takes the LDS tiles where the one-launch layout would
+            if (PART == 2 && tile_lds()) stage_kernel<R, DX, DZ, 0, true, PART><<<blocks, THREADS, 0, s>>>(a);
+            else stage_kernel<R, DX, DZ, 0, false, PART><<<blocks, THREADS, 0, s>>>(a);
+            break;
         case 1: stage_kernel<R, DX, DZ, 1, false, PART><<<blocks, THREADS, 0, s>>>(a); break;
         case 2: stage_kernel<R, DX, DZ, 2, false, PART><<<blocks, THREADS, 0, s>>>(a); break;
         default: stage_kernel<R, DX, DZ, 3, false, PART><<<blocks, THREADS, 0, s>>>(a); break;
@@ -440,7 +444,7 @@ template <class R> struct Engine2D {
     template <int DX, int DZ> void launch_both(const StageArgs<R> &a, int slot) {
         const int blocks = a.gblocks + nInterior;
         if (slot == 0) mark(0, true);
-        if (split_border() && !tile_lds() && a.gblocks > 0 && nInterior > 0) {
+        if (split_border() && a.gblocks > 0 && nInterior > 0) {
             // border blocks on the side stream beside the interior tiles; the next launch waits for both
             check(hipEventRecord(fork[slot], stream), "2D fork");
             check(hipStreamWaitEvent(side, fork[slot], 0), "2D fork");
