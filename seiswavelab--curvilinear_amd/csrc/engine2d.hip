@@ -467,14 +467,27 @@ __device__ inline void merge(Partial &a, const Partial &b) {
     a.traction = fmax(a.traction, b.traction);
     a.bad |= b.bad;
 }
+// Block reduction of the inspection partials: a butterfly over each wave's 64 lanes, then thread 0 merges the
+// waves' results (fixed order, so the same inputs give the same sum); sh[0] holds the block's partial afterwards.
+constexpr int kShSlots = THREADS / 64;
 __device__ void block_reduce(Partial &p, Partial *sh) {
-    const int t = threadIdx.x;
-    sh[t] = p;
-    __syncthreads();
-    for (int s = THREADS / 2; s > 0; s >>= 1) {
-        if (t < s) merge(sh[t], sh[t + s]);
-        __syncthreads();
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        Partial q;
+        q.energy = __shfl_xor(p.energy, o);
+        q.velocity = __shfl_xor(p.velocity, o);
+        q.traction = __shfl_xor(p.traction, o);
+        q.bad = __shfl_xor(p.bad, o);
+        merge(p, q);
     }
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x / 64] = p;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Partial r = sh[0];
+        for (int w = 1; w < kShSlots; ++w) merge(r, sh[w]);
+        sh[0] = r;
+    }
+    __syncthreads();
 }
 // energy (PointKernels2D.hpp:181-184) and inspection terms (CurvilinearSolver2D.cpp:102-111) of one node
 // FAST: one FP64 division (xz^2/(2mu) = xz^2 * 4(lambda+mu) / (8 mu (lambda+mu))), same value to rounding
@@ -705,9 +718,9 @@ __device__ __forceinline__ void combine_field(const StageArgs<R> &A, size_t o, b
 // halo rows of the z-stencil (8 rows) and HV halo vectors on each side of the 64 lane vectors are loaded
 // once into LDS (double-buffered over the fields); the lanes then read their x- and z-taps from LDS.
 template <class R> struct Tile {
-    static constexpr int NV = Lane<R>::N, HV = (3 + NV - 1) / NV, W = 64 + 2 * HV, ROWS = 8;
+    static constexpr int NV = Lane<R>::N, HV = (3 + NV - 1) / NV, W = 64 + 2 * HV, ROWS = 8, BUFS = 2;
     using V = typename Lane<R>::V;
-    V v[2][ROWS][W];
+    V v[BUFS][ROWS][W];
 };
 // One field's stencil taps from an LDS tile: the 2 HV + 1 vectors of the lane's row (x) and the four other rows of
 // its column (z), read by one block of ds_read_b128 that waits for them itself.  The reads are written out so that
@@ -759,7 +772,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
         constexpr int rlo = DZ ? -3 : -1; // first staged row relative to bk
         const size_t cnt = A.count;
         // Field f's tile (rows bk + rlo .. bk + rlo + 7, vectors bx - HV NV .. bx + (64 + HV) NV) goes straight from
-        // global memory into LDS buffer f & 1 (global_load_lds, 16 bytes per lane; lane L of an instruction writes
+        // global memory into LDS buffer f % 2 (global_load_lds, 16 bytes per lane; lane L of an instruction writes
         // slot L after the wave-uniform LDS address).  Wave w loads rows 2w and 2w + 1: one instruction per row for
         // the 64 lane vectors, one for each side's HV halo vectors (lanes 0 .. HV - 1 only).  Every wave issues
         // exactly DMA_PER_FIELD instructions per field, whatever the grid edge: addresses of slots that only dead
@@ -772,7 +785,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
         const auto clamp_x = [&](int ix) { return ix < -m.off ? -m.off : ix + NV > m.ld - m.off ? m.ld - m.off - NV : ix; };
         const auto issue = [&](int f) {
             const R *fp = A.in + (size_t)f * cnt;
-            auto &buf = tile->v[f & 1];
+            auto &buf = tile->v[f % T::BUFS];
 #pragma unroll
             for (int rr = 0; rr < 2; ++rr) {
                 const int r = 2 * wrow + rr, kr = clamp_row(bk + rlo + r);
@@ -786,11 +799,10 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
         // vmcnt / expcnt / lgkmcnt fields of s_waitcnt (gfx9 encoding); the unused counters at their maximum
         constexpr int kWaitNextField = (DMA_PER_FIELD & 15) | ((DMA_PER_FIELD >> 4) << 14) | (7 << 4) | (15 << 8);
         constexpr int kWaitAllLoads = (7 << 4) | (15 << 8);
-        constexpr int kWaitLds = 15 | (3 << 14) | (7 << 4); // lgkmcnt(0)
         issue(0);
 #pragma unroll
         for (int f = 0; f < 5; ++f) {
-            auto &buf = tile->v[f & 1];
+            auto &buf = tile->v[f % T::BUFS];
             if (f + 1 < 5) {
                 issue(f + 1); // into the other buffer, released by the barrier that ended field f - 1
                 __builtin_amdgcn_s_waitcnt(kWaitNextField); // this wave's field-f loads have landed
@@ -828,8 +840,7 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
                     eta[f][j] = over_h<R, kFlatLoads<MECH, LAT>>(r, m.h, m.hinv);
                 }
             }
-            __builtin_amdgcn_s_waitcnt(kWaitLds); // this wave's reads of buffer f & 1 have returned
-            __builtin_amdgcn_s_barrier();         // before any wave refills it with field f + 2
+            __builtin_amdgcn_s_barrier(); // every wave's reads of this buffer have returned before it is refilled
         }
     }
     if (live) {
@@ -1369,7 +1380,7 @@ template <class R> __device__ inline void block_role(const StageArgs<R> &A, int 
 template <class R, int DX, int DZ, int MECH, bool TILE = false, int PART = 0, bool LAT = false>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<R, MECH, LAT>))) void stage_kernel(
     StageArgs<R> A) {
-    __shared__ Partial sh[THREADS];
+    __shared__ Partial sh[kShSlots];
     if (*A.status) return;
     const int b = PART == 2 ? (int)blockIdx.x + A.gblocks : (int)blockIdx.x;
     // block 0 of the launch is dispatched first: the previous step's end overlaps the rest of it
@@ -1405,7 +1416,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(kWavesP
 // Fixed-partition reduction of the per-tile partials (deterministic: same order every run).
 __global__ __launch_bounds__(THREADS) void reduce_kernel(const Partial *partials, int n, Partial *level2,
                                                          const int *status) {
-    __shared__ Partial sh[THREADS];
+    __shared__ Partial sh[kShSlots];
     if (*status) return;
     const int per = (n + gridDim.x - 1) / gridDim.x, begin = blockIdx.x * per, end = min(n, begin + per);
     Partial p{0, 0, 0, 0, 0};
@@ -1456,7 +1467,7 @@ __device__ __forceinline__ void batch_item(const StageArgs<R> *__restrict__ args
 template <class R, int DX, int DZ, int MECH, int PERSIST>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<R, MECH>))) void batch_stage_kernel(
     const StageArgs<R> *__restrict__ args, const int2 *__restrict__ work, int nwork) {
-    __shared__ Partial sh[THREADS];
+    __shared__ Partial sh[kShSlots];
     if constexpr (!PERSIST) {
         batch_item<R, DX, DZ, MECH>(args, work[blockIdx.x], sh);
     } else {
@@ -1489,7 +1500,7 @@ __global__ void batch_begin_kernel(int *const *status, int *const *accepted, dou
 }
 
 template <class R> __device__ void finalize_body(const FinalArgs<R> &A) {
-    __shared__ Partial sh[THREADS];
+    __shared__ Partial sh[kShSlots];
     __shared__ int verdict;
     if (*A.status) return;
     Partial p{0, 0, 0, 0, 0};
@@ -1506,7 +1517,7 @@ template <class R> __device__ void finalize_body(const FinalArgs<R> &A) {
 template <class R>
 __global__ __launch_bounds__(THREADS) void stats_kernel(DMesh<R> m, const R *w, const R *pml, size_t count, Partial *partials,
                                                         int ntx, int ntiles) {
-    __shared__ Partial sh[THREADS];
+    __shared__ Partial sh[kShSlots];
     const int t = tile_of_block(blockIdx.x, ntiles);
     const int i = 3 + (t % ntx) * TX + (threadIdx.x % TX), k = 3 + (t / ntx) * TZ + (threadIdx.x / TX);
     Partial p{0, 0, 0, 0, 0};
