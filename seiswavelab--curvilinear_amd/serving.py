@@ -73,15 +73,19 @@ class ServeReport:
 class Server:
     """budget_bytes: HBM admission budget; chunk: steps between admission / completion checks; options: engine
     switches (Simulation.set_option) applied to every session when it is built; batch_options: runner switches
-    (Batch2D.set_option) of the batch the sessions join."""
+    (Batch2D.set_option) of the batch the sessions join; close_in_background: finished sessions are closed on the
+    builder threads (freeing device memory waits for the device, which would otherwise stall the loop between
+    chunks; two threads of their own, so that closes never queue behind builds); the run's wall time still includes
+    every close."""
 
     def __init__(self, device: int = 0, budget_bytes: float = 200e9, chunk: int = 64, builders: int = 4,
                  keep_traces: bool = True, largest_first: bool = True, options: Dict[str, int] = None,
-                 batch_options: Dict[str, int] = None):
+                 batch_options: Dict[str, int] = None, close_in_background: bool = True):
         self.device, self.budget, self.chunk, self.builders = device, budget_bytes, chunk, builders
         self.keep_traces = keep_traces
         self.options = dict(options or {})
         self.batch_options = dict(batch_options or {})
+        self.close_in_background = close_in_background
         # requests admitted together (a backlog) are built largest first, so that the last sessions to join
         # the batch are the quick ones; admission itself stays in arrival order
         self.largest_first = largest_first
@@ -103,8 +107,10 @@ class Server:
         for name, value in self.batch_options.items():
             batch.set_option(name, value)
         pool = cf.ThreadPoolExecutor(max_workers=self.builders)
+        closer = cf.ThreadPoolExecutor(max_workers=2)
         queue = list(reqs)  # not yet admitted, in arrival order
         building: Dict[int, cf.Future] = {}
+        closing: List[cf.Future] = []
         running: Dict[int, Simulation] = {}
         used, self.max_running, self.max_used = 0, 0, 0
         offered = 0.0
@@ -158,16 +164,22 @@ class Server:
                     if self.keep_traces:
                         r.traces = s.receiver_arrays()
                     batch.remove(s)
-                    s.close()
+                    if self.close_in_background:
+                        closing.append(closer.submit(s.close))
+                    else:
+                        s.close()
                     used -= r.footprint
             elif building:
                 time.sleep(0.0005)
             elif queue:
                 time.sleep(max(0.0, min(0.002, queue[0][1] - now())))
+        for f in closing:
+            f.result()
         wall = now()
         stats = batch.stats()
         batch.close()
         pool.shutdown(wait=True)
+        closer.shutdown(wait=True)
         return ServeReport([results[i] for i, _, _ in reqs], wall, offered, stats)
 
 
