@@ -38,9 +38,10 @@ def alone(p):
     return a
 
 
-def test_each_result_is_the_same_alone_and_in_the_mix():
+@pytest.mark.parametrize("close_in_background", [True, False])
+def test_each_result_is_the_same_alone_and_in_the_mix(close_in_background):
     reqs = small_requests()
-    report = Server(chunk=16).run(copy.deepcopy(reqs))
+    report = Server(chunk=16, close_in_background=close_in_background).run(copy.deepcopy(reqs))
     assert all(not r.error for r in report.results)
     for r, (i, _, p) in zip(report.results, reqs):
         ref = alone(p)
