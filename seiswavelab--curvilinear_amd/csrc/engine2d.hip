@@ -773,11 +773,14 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
         const size_t cnt = A.count;
         // Field f's tile (rows bk + rlo .. bk + rlo + 7, vectors bx - HV NV .. bx + (64 + HV) NV) goes straight from
         // global memory into LDS buffer f % 2 (global_load_lds, 16 bytes per lane; lane L of an instruction writes
-        // slot L after the wave-uniform LDS address).  Wave w loads rows 2w and 2w + 1: one instruction per row for
-        // the 64 lane vectors, one for each side's HV halo vectors (lanes 0 .. HV - 1 only).  Every wave issues
-        // exactly DMA_PER_FIELD instructions per field, whatever the grid edge: addresses of slots that only dead
-        // lanes read are clamped into the row instead of skipped, so the counter arithmetic below is exact.
-        constexpr int DMA_PER_FIELD = 6;
+        // slot L after the wave-uniform LDS address).  Wave w loads rows 2w and 2w + 1, which are 2 W contiguous
+        // slots (halo vectors included): lane L of instruction c fills slot 64 c + L of them, i.e. row 2w + s / W,
+        // vector s % W (s = 64 c + L), so three instructions cover both rows (the third with the remaining
+        // 2 W - 128 lanes).  Every wave issues exactly DMA_PER_FIELD instructions per field, whatever the grid
+        // edge: addresses of slots that only dead lanes read are clamped into the row instead of skipped, so the
+        // counter arithmetic below is exact.
+        constexpr int SPAN = 2 * T::W, DMA_PER_FIELD = (SPAN + 63) / 64;
+        static_assert(DMA_PER_FIELD == 3, "two rows of a tile take three 64-lane loads");
         const auto dma = [&](const R *g, V *l) {
             __builtin_amdgcn_global_load_lds(static_cast<const void *>(g), (__attribute__((address_space(3))) void *)l, 16, 0, 0);
         };
@@ -786,14 +789,12 @@ __device__ __forceinline__ void interior_body(const StageArgs<R> &A, int tl, Par
         const auto issue = [&](int f) {
             const R *fp = A.in + (size_t)f * cnt;
             auto &buf = tile->v[f % T::BUFS];
+            V *first = &buf[2 * wrow][0]; // the wave's 2 W slots, contiguous
 #pragma unroll
-            for (int rr = 0; rr < 2; ++rr) {
-                const int r = 2 * wrow + rr, kr = clamp_row(bk + rlo + r);
-                dma(fp + at(m, clamp_x(bx + lane * NV), kr), &buf[r][T::HV]);
-                if (lane < T::HV) {
-                    dma(fp + at(m, clamp_x(bx + (lane - T::HV) * NV), kr), &buf[r][0]);
-                    dma(fp + at(m, clamp_x(bx + (64 + lane) * NV), kr), &buf[r][T::HV + 64]);
-                }
+            for (int c = 0; c < DMA_PER_FIELD; ++c) {
+                const int sl = 64 * c + lane, r = 2 * wrow + sl / T::W, v = sl % T::W;
+                const R *g = fp + at(m, clamp_x(bx + (v - T::HV) * NV), clamp_row(bk + rlo + r));
+                if (64 * c + 63 < SPAN || lane < SPAN - 64 * c) dma(g, first + 64 * c);
             }
         };
         // vmcnt / expcnt / lgkmcnt fields of s_waitcnt (gfx9 encoding); the unused counters at their maximum
